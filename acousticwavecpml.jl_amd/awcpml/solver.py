@@ -1,0 +1,256 @@
+"""Drop-in solve2D / solve3D over libawcpml, and the Plan wrapper.
+
+``solve2D``/``solve3D`` keep the reference's entry points, keyword arguments,
+outputs and errors (src/shared/acoustic_2D_xPU.jl:133-444,
+src/shared/acoustic_3D_xPU.jl:169-544): the host setup runs in FP64 on the
+CPU (``host.setup``), the time loop runs on the GPU through the C ABI, the
+final pressure field is returned and ``recs.seismograms`` is filled.
+"""
+import ctypes
+import os
+import time
+
+import numpy as np
+
+from . import _lib
+from .host import setup
+
+DTYPES = {"f64": _lib.F64, "float64": _lib.F64, np.float64: _lib.F64,
+          "f32": _lib.F32, "float32": _lib.F32, np.float32: _lib.F32}
+
+
+def _dtype_code(dtype):
+    try:
+        return DTYPES[dtype]
+    except (KeyError, TypeError):
+        raise ValueError(f"unsupported dtype {dtype!r} (use 'f64' or 'f32')")
+
+
+class Plan:
+    """One velocity model with ``nshots`` independent shots on the GPU.
+
+    Thin owner of an ``awcpml_plan`` handle; see include/awcpml.h for the
+    semantics of each call.
+    """
+
+    def __init__(self, n, halo, dtype="f64", nshots=1):
+        L = _lib.load()
+        self.n = tuple(int(v) for v in n)
+        self.ndim = len(self.n)
+        self.halo = int(halo)
+        self.nshots = int(nshots)
+        self.dtype = _dtype_code(dtype)
+        nz = self.n[2] if self.ndim == 3 else 1
+        h = ctypes.c_void_p()
+        _lib.check(L.awcpml_plan_create(self.ndim, self.n[0], self.n[1], nz, self.halo,
+                                        self.dtype, self.nshots, ctypes.byref(h)))
+        self._h = h
+        self.nt = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().awcpml_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- configuration ------------------------------------------------
+    def set_model(self, fact, inv_d, inv_d2):
+        fact = np.asfortranarray(np.asarray(fact, dtype=np.float64))
+        assert fact.shape == self.n, (fact.shape, self.n)
+        inv_d = np.ascontiguousarray(inv_d, dtype=np.float64)
+        inv_d2 = np.ascontiguousarray(inv_d2, dtype=np.float64)
+        _lib.check(_lib.load().awcpml_set_model(self._h, fact.ctypes.data_as(_lib._dp),
+                                                _lib.dptr(inv_d), _lib.dptr(inv_d2)))
+
+    def set_cpml(self, axis, coefs):
+        """coefs: [a_l, a_r, b_K_l, b_K_r, a_hl, a_hr, b_K_hl, b_K_hr]."""
+        arrs = []
+        for c, v in enumerate(coefs):
+            v = np.ascontiguousarray(v, dtype=np.float64)
+            need = self.halo + (1 if c >= 4 else 0)
+            assert v.size == need, (c, v.size, need)
+            arrs.append(v if v.size else np.zeros(1))
+        _lib.check(_lib.load().awcpml_set_cpml(self._h, axis, *[_lib.dptr(a) for a in arrs]))
+
+    def set_sources(self, shot, pos, dt2srctf):
+        pos = np.asfortranarray(np.asarray(pos, dtype=np.int64).reshape(-1, self.ndim))
+        wav = np.asfortranarray(np.asarray(dt2srctf, dtype=np.float64))
+        nt = wav.shape[0]
+        assert wav.shape[1] == pos.shape[0]
+        _lib.check(_lib.load().awcpml_set_sources(self._h, shot, pos.shape[0],
+                                                  pos.ctypes.data_as(_lib._i64p), nt,
+                                                  wav.ctypes.data_as(_lib._dp)))
+        self.nt = nt
+
+    def set_receivers(self, shot, pos):
+        pos = np.asfortranarray(np.asarray(pos, dtype=np.int64).reshape(-1, self.ndim))
+        _lib.check(_lib.load().awcpml_set_receivers(self._h, shot, pos.shape[0],
+                                                    pos.ctypes.data_as(_lib._i64p)))
+        if not hasattr(self, "_nrecs"):
+            self._nrecs = {}
+        self._nrecs[shot] = pos.shape[0]
+
+    def set_tiling(self, block_y=0, march_len=0):
+        _lib.check(_lib.load().awcpml_set_tiling(self._h, block_y, march_len))
+
+    def tiling(self):
+        by, ml = ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.load().awcpml_get_tiling(self._h, ctypes.byref(by), ctypes.byref(ml)))
+        return by.value, ml.value
+
+    # -- execution ----------------------------------------------------
+    def reset(self):
+        _lib.check(_lib.load().awcpml_reset(self._h))
+
+    def advance(self, nsteps):
+        _lib.check(_lib.load().awcpml_advance(self._h, int(nsteps)))
+
+    def forward(self, it):
+        _lib.check(_lib.load().awcpml_forward(self._h, int(it)))
+
+    def steps_done(self):
+        v = ctypes.c_int64()
+        _lib.check(_lib.load().awcpml_steps_done(self._h, ctypes.byref(v)))
+        return v.value
+
+    def traces(self, shot=0):
+        nrec = self._nrecs[shot]
+        out = np.zeros((self.nt, nrec), order="F")
+        _lib.check(_lib.load().awcpml_get_traces(self._h, shot, out.ctypes.data_as(_lib._dp)))
+        return out
+
+    def field(self, shot=0, which=1):
+        out = np.zeros(self.n, order="F")
+        _lib.check(_lib.load().awcpml_get_field(self._h, shot, which, out.ctypes.data_as(_lib._dp)))
+        return out
+
+    def time_stencil(self, nlaunches):
+        ms = ctypes.c_double()
+        _lib.check(_lib.load().awcpml_time_stencil(self._h, int(nlaunches), ctypes.byref(ms)))
+        return ms.value
+
+    def time_advance(self, nsteps):
+        ms = ctypes.c_double()
+        _lib.check(_lib.load().awcpml_time_advance(self._h, int(nsteps), ctypes.byref(ms)))
+        return ms.value
+
+    @classmethod
+    def from_setup(cls, S, dtype="f64", nshots=1):
+        """Plan with the model and CPML of a host ``Setup``; shot 0 gets its
+        sources and receivers."""
+        P = cls(S.n, S.halo, dtype=dtype, nshots=nshots)
+        P.set_model(S.fact, S.inv_d, S.inv_d2)
+        for a in range(S.ndim):
+            P.set_cpml(a, S.coef[a])
+        P.set_sources(0, S.possrcs, S.dt2srctf)
+        P.set_receivers(0, S.posrecs)
+        return P
+
+
+def _effective_bytes(n, halo, esize):
+    """A_eff of the reference's do_bench report (acoustic_2D_xPU.jl:288-294,
+    acoustic_3D_xPU.jl:341-349), in GB."""
+    if len(n) == 2:
+        nx, ny = n
+        elems = ((halo + 1) * ny * 2 * 3 + (halo + 1) * nx * 2 * 3 + (halo + 1) * ny * 2 * 2
+                 + (halo + 1) * nx * 2 * 2 + 4 * nx * ny)
+    else:
+        nx, ny, nz = n
+        elems = ((halo + 1) * ny * nz * 2 * 3 + (halo + 1) * nx * nz * 2 * 3
+                 + (halo + 1) * nx * ny * 2 * 3 + (halo + 1) * ny * nz * 2 * 2
+                 + (halo + 1) * nx * nz * 2 * 2 + (halo + 1) * nx * ny * 2 * 2 + 4 * nx * ny * nz)
+    return elems * esize / 1e9
+
+
+def _bench(P, n, halo, esize, trials=200):
+    """do_bench: minimum over repeated single forward! calls with it = 1
+    (acoustic_2D_xPU.jl:261-299), printed in the reference's format."""
+    times = []
+    P.forward(1)
+    for _ in range(trials):
+        t0 = time.perf_counter()
+        P.forward(1)
+        times.append(time.perf_counter() - t0)
+    t_it = min(times)
+    a_eff = _effective_bytes(n, halo, esize)
+    size = "x".join(str(v) for v in n)
+    print("size = %s, time = %1.3e sec, Teff = %1.3f GB/s" % (size, t_it, a_eff / t_it))
+    return t_it
+
+
+def _solve(lengths, lt, vel, srcs, recs, halo, rcoef, ppw, freetop, do_bench, do_vis,
+           dtype, do_save=False, nsave=5, save_name="acoustic3D_xPU", save_dir=None):
+    if do_vis:
+        raise NotImplementedError("do_vis (Plots.jl animations) is not part of this library")
+    S = setup(lengths, lt, vel, srcs, recs, halo=halo, rcoef=rcoef, ppw=ppw, freetop=freetop)
+    P = Plan.from_setup(S, dtype=dtype)
+    try:
+        if do_bench:
+            _bench(P, S.n, S.halo, 8 if P.dtype == _lib.F64 else 4)
+            return None
+        if do_save:
+            save_dir = save_dir or os.path.join(os.getcwd(), "simulations", "tmp")
+            os.makedirs(save_dir, exist_ok=True)
+            done = 0
+            while done < S.nt:
+                step = min(nsave - (done % nsave), S.nt - done)
+                P.advance(step)
+                done += step
+                if done % nsave == 0:
+                    _save_snapshot(save_dir, save_name, done, lengths, S, P.field(0, 1))
+        else:
+            P.advance(S.nt)
+        recs.seismograms = P.traces(0)
+        return P.field(0, 1)
+    finally:
+        P.close()
+
+
+def _save_snapshot(save_dir, save_name, it, lengths, S, pcur):
+    """do_save snapshot with the fields of acoustic_3D_xPU.jl:480-498
+    (lx, ly, lz, halo, pcur, possrcs, posrecs), written as .npz because no
+    HDF5 library is available here."""
+    path = os.path.join(save_dir, f"{save_name}_it{it}.npz")
+    if os.path.exists(path):
+        os.remove(path)
+    np.savez(path, lx=lengths[0], ly=lengths[1], lz=lengths[2] if len(lengths) > 2 else 0.0,
+             halo=S.halo, pcur=pcur, possrcs=S.possrcs, posrecs=S.posrecs)
+
+
+def solve2D(lx, ly, lt, vel, srcs, recs, *, halo=20, rcoef=0.0001, ppw=10.0, freetop=True,
+            do_bench=False, do_vis=False, nvis=5, gif_name="acoustic2D_xPU", plims=(-1, 1),
+            threshold=0.01, dtype="f64"):
+    """2D CPML acoustic solve (src/shared/acoustic_2D_xPU.jl:133-444).
+
+    Returns the final pressure field (nx, ny) and fills ``recs.seismograms``
+    with the (nt, nrecs) traces.  ``dtype="f32"`` runs the FP32 extension.
+    """
+    vel = np.asarray(vel)
+    assert vel.ndim == 2
+    return _solve((lx, ly), lt, vel, srcs, recs, halo, rcoef, ppw, freetop, do_bench, do_vis,
+                  dtype)
+
+
+def solve3D(lx, ly, lz, lt, vel, srcs, recs, *, halo=20, rcoef=0.0001, ppw=10.0, freetop=True,
+            do_bench=False, do_vis=False, do_save=False, nvis=5, nsave=5,
+            gif_name="acoustic3D_xPU_slice", save_name="acoustic3D_xPU", plims=(-1.0, 1.0),
+            threshold=0.01, dtype="f64", save_dir=None):
+    """3D CPML acoustic solve (src/shared/acoustic_3D_xPU.jl:169-544).
+
+    Returns the final pressure field (nx, ny, nz) and fills
+    ``recs.seismograms``.  ``do_save`` writes a snapshot of pcur every
+    ``nsave`` steps to ``save_dir`` (default ./simulations/tmp).
+    """
+    vel = np.asarray(vel)
+    assert vel.ndim == 3
+    return _solve((lx, ly, lz), lt, vel, srcs, recs, halo, rcoef, ppw, freetop, do_bench,
+                  do_vis, dtype, do_save=do_save, nsave=nsave, save_name=save_name,
+                  save_dir=save_dir)

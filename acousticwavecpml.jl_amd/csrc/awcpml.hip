@@ -1,0 +1,711 @@
+// awcpml.hip -- plan management and launches behind include/awcpml.h.
+//
+// Device layout of a plan (per shot; shots are stacked back to back):
+//   fields pold/pcur/pnew: px x ny x nz, px = nx rounded up to 128 bytes,
+//     x fastest (the reference's column-major order, rows aligned);
+//   fact: one px x ny x nz array shared by all shots;
+//   psi_x: (halo+1) x ny x nz, psi_y: px x (halo+1) x nz,
+//   psi_z: px x ny x (halo+1), two copies each (step n-1 / step n);
+//   xi_*: the same with halo layers, one copy (updated in place);
+//   CPML coefficient vectors, wavelets (nt per source), traces (nt per
+//     receiver), all in the plan's precision.
+// The three field buffers rotate (pold, pcur, pnew) <- (pcur, pnew, pold)
+// after every step, as the reference's time loop does
+// (src/shared/acoustic_2D_xPU.jl:332); Dirichlet nodes are never written
+// by the stencil, so a source on such a node accumulates per buffer exactly
+// as in the reference.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "awcpml.h"
+#include "stencil_kernels.h"
+
+using awcpml::StencilArgs;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess)                                                      \
+            return fail(e_ == hipErrorOutOfMemory ? AWCPML_ERR_NOMEM : AWCPML_ERR_HIP, \
+                        "%s failed: %s", #expr, hipGetErrorString(e_));            \
+    } while (0)
+
+int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+}  // namespace
+
+struct awcpml_plan {
+    int ndim = 0, dtype = 0, nshots = 0, halo = 0;
+    int64_t n[3] = {1, 1, 1};
+    int64_t px = 0;
+    size_t esize = 0;
+    int64_t field_elems = 0;
+    void* p[3] = {nullptr, nullptr, nullptr};
+    void* fact = nullptr;
+    void* psi[3][2][2] = {};
+    void* xi[3][2] = {};
+    int64_t psi_elems[3] = {0, 0, 0};
+    int64_t xi_elems[3] = {0, 0, 0};
+    void* coef[3][8] = {};
+    double inv_d[3] = {0, 0, 0}, inv_d2[3] = {0, 0, 0};
+    int active[3][2] = {};
+    bool model_set = false;
+    bool cpml_set[3] = {false, false, false};
+
+    // sources / receivers (host staging, per shot)
+    std::vector<std::vector<int64_t>> src_lin, rec_lin;
+    std::vector<std::vector<double>> src_wav;  // nt per source
+    std::vector<bool> src_set, rec_set;
+    int64_t nt = -1;
+    bool io_dirty = true;
+    void* d_src_lin = nullptr;
+    void* d_src_wav = nullptr;
+    void* d_src_start = nullptr;
+    void* d_src_seq = nullptr;
+    void* d_rec_lin = nullptr;
+    void* d_rec_start = nullptr;
+    void* d_traces = nullptr;
+    int64_t n_src_total = 0, n_rec_total = 0;
+    bool has_io = false;
+
+    int64_t it = 0;   // steps done
+    int rot = 0;      // p[rot] = pold, p[rot+1] = pcur, p[rot+2] = pnew
+    int par = 0;      // psi[..][par] = previous step
+
+    hipStream_t stream = nullptr;
+    int block_y = 0, march_len = 0;
+    int device = 0;
+};
+
+namespace {
+
+int64_t slab_elems(const awcpml_plan* P, int axis, int64_t layers) {
+    // layouts documented at the top of the file
+    if (axis == 0) return layers * P->n[1] * P->n[2];
+    if (axis == 1) return P->px * layers * P->n[2];
+    return P->px * P->n[1] * layers;
+}
+
+int dalloc(void** ptr, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    HIP_TRY(hipMalloc(ptr, bytes));
+    return AWCPML_OK;
+}
+
+void dfree(void*& ptr) {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+}
+
+// host FP64 values -> device buffer of the plan's precision
+int upload(awcpml_plan* P, void* dst, const double* src, size_t count) {
+    if (count == 0) return AWCPML_OK;
+    if (P->dtype == AWCPML_F64) {
+        HIP_TRY(hipMemcpyAsync(dst, src, count * 8, hipMemcpyHostToDevice, P->stream));
+    } else {
+        std::vector<float> tmp(count);
+        for (size_t i = 0; i < count; ++i) tmp[i] = (float)src[i];
+        HIP_TRY(hipMemcpyAsync(dst, tmp.data(), count * 4, hipMemcpyHostToDevice, P->stream));
+        HIP_TRY(hipStreamSynchronize(P->stream));
+    }
+    return AWCPML_OK;
+}
+
+int check_plan(const awcpml_plan* P) {
+    if (!P) return fail(AWCPML_ERR_ARG, "plan is NULL");
+    return AWCPML_OK;
+}
+
+int default_tiling(awcpml_plan* P) {
+    if (P->block_y <= 0) P->block_y = 4;
+    if (P->march_len <= 0) {
+        if (P->ndim == 3) {
+            // enough blocks to fill 256 CUs several times over
+            const int64_t tiles = round_up(P->n[0], awcpml::kTileX) / awcpml::kTileX *
+                                  ((P->n[1] - 2 + P->block_y - 1) / P->block_y) * P->nshots;
+            int64_t interior = P->n[2] - 2;
+            int64_t want_chunks = std::max<int64_t>(1, (4096 + tiles - 1) / tiles);
+            int64_t len = (interior + want_chunks - 1) / want_chunks;
+            P->march_len = (int)std::max<int64_t>(16, std::min<int64_t>(len, 128));
+        } else {
+            int64_t interior = P->n[1] - 2;
+            int64_t len = std::max<int64_t>(8, std::min<int64_t>(64, interior / 16));
+            P->march_len = (int)len;
+        }
+    }
+    return AWCPML_OK;
+}
+
+template <typename T>
+StencilArgs<T> make_args(const awcpml_plan* P) {
+    StencilArgs<T> A;
+    memset(&A, 0, sizeof A);
+    A.nx = P->n[0];
+    A.ny = P->n[1];
+    A.nz = P->n[2];
+    A.px = P->px;
+    A.halo = P->halo;
+    A.field_stride = P->field_elems;
+    A.pold = (const T*)P->p[P->rot % 3];
+    A.pcur = (const T*)P->p[(P->rot + 1) % 3];
+    A.pnew = (T*)P->p[(P->rot + 2) % 3];
+    A.fact = (const T*)P->fact;
+    for (int a = 0; a < 3; ++a) {
+        A.inv_d[a] = (T)P->inv_d[a];
+        A.inv_d2[a] = (T)P->inv_d2[a];
+        A.psi_stride[a] = P->psi_elems[a];
+        A.xi_stride[a] = P->xi_elems[a];
+        for (int s = 0; s < 2; ++s) {
+            A.active[a][s] = (a < P->ndim) ? P->active[a][s] : 0;
+            awcpml::CpmlSide<T>& S = A.side[a][s];
+            S.psi_rd = (const T*)P->psi[a][s][P->par];
+            S.psi_wr = (T*)P->psi[a][s][P->par ^ 1];
+            S.xi = (T*)P->xi[a][s];
+            S.a = (const T*)P->coef[a][0 + s];
+            S.bK = (const T*)P->coef[a][2 + s];
+            S.ah = (const T*)P->coef[a][4 + s];
+            S.bKh = (const T*)P->coef[a][6 + s];
+        }
+    }
+    const int by = P->block_y;
+    A.march_len = P->march_len;
+    A.nshots = P->nshots;
+    A.tiles_x = (int)(round_up(P->n[0], awcpml::kTileX) / awcpml::kTileX);
+    if (P->ndim == 3) {
+        A.tiles_y = (int)((P->n[1] - 2 + by - 1) / by);
+        A.nchunks = (int)((P->n[2] - 2 + P->march_len - 1) / P->march_len);
+        A.logical_blocks = A.nshots * A.tiles_x * A.tiles_y * A.nchunks;
+    } else {
+        A.nchunks = (int)((P->n[1] - 2 + P->march_len - 1) / P->march_len);
+        A.tiles_y = (A.nchunks + by - 1) / by;
+        A.logical_blocks = A.nshots * A.tiles_x * A.tiles_y;
+    }
+    A.grid_blocks = (int)round_up(A.logical_blocks, awcpml::kNumXcd);
+    return A;
+}
+
+template <typename T>
+int launch_stencil_t(awcpml_plan* P) {
+    StencilArgs<T> A = make_args<T>(P);
+    dim3 block(awcpml::kTileX, P->block_y);
+    dim3 grid(A.grid_blocks);
+    if (P->ndim == 3)
+        awcpml::stencil3d<T><<<grid, block, 0, P->stream>>>(A);
+    else
+        awcpml::stencil2d<T><<<grid, block, 0, P->stream>>>(A);
+    HIP_TRY(hipGetLastError());
+    return AWCPML_OK;
+}
+
+int launch_stencil(awcpml_plan* P) {
+    return P->dtype == AWCPML_F64 ? launch_stencil_t<double>(P) : launch_stencil_t<float>(P);
+}
+
+template <typename T>
+int launch_io_t(awcpml_plan* P, int64_t it0) {
+    T* pnew = (T*)P->p[(P->rot + 2) % 3];
+    awcpml::sources_receivers<T><<<dim3(P->nshots), dim3(256), 0, P->stream>>>(
+        pnew, P->field_elems, (const int64_t*)P->d_src_lin, (const T*)P->d_src_wav,
+        (const int*)P->d_src_start, (const int*)P->d_src_seq, (const int64_t*)P->d_rec_lin,
+        (T*)P->d_traces, (const int*)P->d_rec_start, P->nt, it0);
+    HIP_TRY(hipGetLastError());
+    return AWCPML_OK;
+}
+
+int launch_io(awcpml_plan* P, int64_t it0) {
+    return P->dtype == AWCPML_F64 ? launch_io_t<double>(P, it0) : launch_io_t<float>(P, it0);
+}
+
+// Upload the per-shot source/receiver lists as CSR arrays.
+int sync_io(awcpml_plan* P) {
+    if (!P->io_dirty) return AWCPML_OK;
+    for (int s = 0; s < P->nshots; ++s) {
+        if (!P->src_set[s]) return fail(AWCPML_ERR_STATE, "sources of shot %d not set", s);
+        if (!P->rec_set[s]) return fail(AWCPML_ERR_STATE, "receivers of shot %d not set", s);
+    }
+    std::vector<int64_t> slin, rlin;
+    std::vector<int> sstart(1, 0), rstart(1, 0), seq;
+    std::vector<double> wav;
+    for (int s = 0; s < P->nshots; ++s) {
+        const auto& L = P->src_lin[s];
+        slin.insert(slin.end(), L.begin(), L.end());
+        wav.insert(wav.end(), P->src_wav[s].begin(), P->src_wav[s].end());
+        sstart.push_back((int)slin.size());
+        std::vector<int64_t> sorted(L);
+        std::sort(sorted.begin(), sorted.end());
+        seq.push_back(std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end() ? 1 : 0);
+        rlin.insert(rlin.end(), P->rec_lin[s].begin(), P->rec_lin[s].end());
+        rstart.push_back((int)rlin.size());
+    }
+    dfree(P->d_src_lin);
+    dfree(P->d_src_wav);
+    dfree(P->d_src_start);
+    dfree(P->d_src_seq);
+    dfree(P->d_rec_lin);
+    dfree(P->d_rec_start);
+    dfree(P->d_traces);
+    int st;
+    if ((st = dalloc(&P->d_src_lin, slin.size() * 8))) return st;
+    if ((st = dalloc(&P->d_src_wav, wav.size() * P->esize))) return st;
+    if ((st = dalloc(&P->d_src_start, sstart.size() * 4))) return st;
+    if ((st = dalloc(&P->d_src_seq, seq.size() * 4))) return st;
+    if ((st = dalloc(&P->d_rec_lin, rlin.size() * 8))) return st;
+    if ((st = dalloc(&P->d_rec_start, rstart.size() * 4))) return st;
+    if ((st = dalloc(&P->d_traces, (size_t)rlin.size() * (size_t)std::max<int64_t>(P->nt, 1) * P->esize)))
+        return st;
+    if (!slin.empty())
+        HIP_TRY(hipMemcpyAsync(P->d_src_lin, slin.data(), slin.size() * 8, hipMemcpyHostToDevice, P->stream));
+    if ((st = upload(P, P->d_src_wav, wav.data(), wav.size()))) return st;
+    HIP_TRY(hipMemcpyAsync(P->d_src_start, sstart.data(), sstart.size() * 4, hipMemcpyHostToDevice, P->stream));
+    HIP_TRY(hipMemcpyAsync(P->d_src_seq, seq.data(), seq.size() * 4, hipMemcpyHostToDevice, P->stream));
+    if (!rlin.empty())
+        HIP_TRY(hipMemcpyAsync(P->d_rec_lin, rlin.data(), rlin.size() * 8, hipMemcpyHostToDevice, P->stream));
+    HIP_TRY(hipMemcpyAsync(P->d_rec_start, rstart.data(), rstart.size() * 4, hipMemcpyHostToDevice, P->stream));
+    if (!rlin.empty() && P->nt > 0)
+        HIP_TRY(hipMemsetAsync(P->d_traces, 0, (size_t)rlin.size() * P->nt * P->esize, P->stream));
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    P->n_src_total = (int64_t)slin.size();
+    P->n_rec_total = (int64_t)rlin.size();
+    P->has_io = (P->n_src_total + P->n_rec_total) > 0;
+    P->io_dirty = false;
+    return AWCPML_OK;
+}
+
+int ready(awcpml_plan* P) {
+    if (!P->model_set) return fail(AWCPML_ERR_STATE, "model not set (awcpml_set_model)");
+    for (int a = 0; a < P->ndim; ++a)
+        if (P->halo > 0 && !P->cpml_set[a])
+            return fail(AWCPML_ERR_STATE, "CPML coefficients of axis %d not set", a);
+    int st = sync_io(P);
+    if (st) return st;
+    return default_tiling(P);
+}
+
+// one time step with explicit 0-based index it0, then rotate
+int do_step(awcpml_plan* P, int64_t it0) {
+    int st = launch_stencil(P);
+    if (st) return st;
+    if (P->has_io && (st = launch_io(P, it0))) return st;
+    P->rot = (P->rot + 1) % 3;
+    P->par ^= 1;
+    return AWCPML_OK;
+}
+
+int64_t lin_index(const awcpml_plan* P, const int64_t* pos, int count, int item) {
+    int64_t i = pos[item] - 1;
+    int64_t j = pos[(int64_t)count + item] - 1;
+    int64_t k = (P->ndim == 3) ? pos[2 * (int64_t)count + item] - 1 : 0;
+    return i + P->px * (j + P->n[1] * k);
+}
+
+int check_positions(const awcpml_plan* P, const int64_t* pos, int count, const char* what) {
+    for (int r = 0; r < count; ++r)
+        for (int a = 0; a < P->ndim; ++a) {
+            int64_t v = pos[(int64_t)a * count + r];
+            if (v < 1 || v > P->n[a])
+                return fail(AWCPML_ERR_RANGE, "At least one %s is not inside the model! (%s %d axis %d index %lld)",
+                            what, what, r, a, (long long)v);
+        }
+    return AWCPML_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* awcpml_last_error(void) { return g_err.c_str(); }
+
+const char* awcpml_version(void) { return "awcpml 0.1.0 (gfx950)"; }
+
+int awcpml_plan_create(int ndim, int64_t nx, int64_t ny, int64_t nz, int halo, int dtype,
+                       int nshots, awcpml_plan** out) {
+    g_err.clear();
+    if (!out) return fail(AWCPML_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (ndim != 2 && ndim != 3) return fail(AWCPML_ERR_ARG, "ndim must be 2 or 3 (got %d)", ndim);
+    if (dtype != AWCPML_F64 && dtype != AWCPML_F32) return fail(AWCPML_ERR_ARG, "unknown dtype %d", dtype);
+    if (nshots < 1) return fail(AWCPML_ERR_ARG, "nshots must be >= 1");
+    if (halo < 0) return fail(AWCPML_ERR_ARG, "halo must be >= 0");
+    if (ndim == 2) nz = 1;
+    int64_t n[3] = {nx, ny, nz};
+    for (int a = 0; a < ndim; ++a) {
+        if (n[a] < 3) return fail(AWCPML_ERR_ARG, "axis %d has %lld points (need >= 3)", a, (long long)n[a]);
+        if (n[a] < 2 * (int64_t)halo + 2)
+            return fail(AWCPML_ERR_ARG,
+                        "axis %d has %lld points: the two CPML layers of halo %d overlap (need >= %d)", a,
+                        (long long)n[a], halo, 2 * halo + 2);
+    }
+    awcpml_plan* P = new (std::nothrow) awcpml_plan();
+    if (!P) return fail(AWCPML_ERR_NOMEM, "host allocation failed");
+    P->ndim = ndim;
+    P->dtype = dtype;
+    P->nshots = nshots;
+    P->halo = halo;
+    for (int a = 0; a < 3; ++a) P->n[a] = n[a];
+    P->esize = dtype == AWCPML_F64 ? 8 : 4;
+    P->px = round_up(nx, (int64_t)(128 / P->esize));
+    P->field_elems = P->px * ny * nz;
+    P->src_lin.resize(nshots);
+    P->rec_lin.resize(nshots);
+    P->src_wav.resize(nshots);
+    P->src_set.assign(nshots, false);
+    P->rec_set.assign(nshots, false);
+    int st = AWCPML_OK;
+    hipError_t he = hipGetDevice(&P->device);
+    if (he != hipSuccess) {
+        delete P;
+        return fail(AWCPML_ERR_HIP, "hipGetDevice failed: %s", hipGetErrorString(he));
+    }
+    he = hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) {
+        delete P;
+        return fail(AWCPML_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(he));
+    }
+    const size_t fbytes = (size_t)P->field_elems * nshots * P->esize;
+    for (int b = 0; b < 3 && !st; ++b) st = dalloc(&P->p[b], fbytes);
+    if (!st) st = dalloc(&P->fact, (size_t)P->field_elems * P->esize);
+    for (int a = 0; a < ndim && !st; ++a) {
+        P->psi_elems[a] = slab_elems(P, a, halo + 1);
+        P->xi_elems[a] = slab_elems(P, a, halo);
+        for (int s = 0; s < 2 && !st; ++s) {
+            st = dalloc(&P->psi[a][s][0], (size_t)P->psi_elems[a] * nshots * P->esize);
+            if (!st) st = dalloc(&P->psi[a][s][1], (size_t)P->psi_elems[a] * nshots * P->esize);
+            if (!st) st = dalloc(&P->xi[a][s], (size_t)P->xi_elems[a] * nshots * P->esize);
+        }
+        for (int c = 0; c < 8 && !st; ++c) st = dalloc(&P->coef[a][c], (size_t)(halo + 1) * P->esize);
+    }
+    if (!st) {
+        hipError_t e = hipMemsetAsync(P->fact, 0, (size_t)P->field_elems * P->esize, P->stream);
+        if (e != hipSuccess) st = fail(AWCPML_ERR_HIP, "hipMemset failed: %s", hipGetErrorString(e));
+    }
+    if (!st) st = awcpml_reset(P);
+    if (!st) st = default_tiling(P);
+    if (st) {
+        std::string keep = g_err;
+        awcpml_plan_destroy(P);
+        g_err = keep;
+        return st;
+    }
+    *out = P;
+    return AWCPML_OK;
+}
+
+int awcpml_plan_destroy(awcpml_plan* P) {
+    if (!P) return AWCPML_OK;
+    if (P->stream) (void)hipStreamSynchronize(P->stream);
+    for (int b = 0; b < 3; ++b) dfree(P->p[b]);
+    dfree(P->fact);
+    for (int a = 0; a < 3; ++a) {
+        for (int s = 0; s < 2; ++s) {
+            dfree(P->psi[a][s][0]);
+            dfree(P->psi[a][s][1]);
+            dfree(P->xi[a][s]);
+        }
+        for (int c = 0; c < 8; ++c) dfree(P->coef[a][c]);
+    }
+    dfree(P->d_src_lin);
+    dfree(P->d_src_wav);
+    dfree(P->d_src_start);
+    dfree(P->d_src_seq);
+    dfree(P->d_rec_lin);
+    dfree(P->d_rec_start);
+    dfree(P->d_traces);
+    if (P->stream) (void)hipStreamDestroy(P->stream);
+    delete P;
+    return AWCPML_OK;
+}
+
+int awcpml_set_model(awcpml_plan* P, const double* fact, const double* inv_d, const double* inv_d2) {
+    g_err.clear();
+    int st = check_plan(P);
+    if (st) return st;
+    if (!fact || !inv_d || !inv_d2) return fail(AWCPML_ERR_ARG, "NULL argument");
+    for (int a = 0; a < P->ndim; ++a) {
+        P->inv_d[a] = inv_d[a];
+        P->inv_d2[a] = inv_d2[a];
+    }
+    const int64_t nx = P->n[0], rows = P->n[1] * P->n[2];
+    if (P->dtype == AWCPML_F64) {
+        HIP_TRY(hipMemcpy2DAsync(P->fact, P->px * 8, fact, nx * 8, nx * 8, rows, hipMemcpyHostToDevice,
+                                 P->stream));
+    } else {
+        std::vector<float> tmp((size_t)nx * rows);
+        for (size_t q = 0; q < tmp.size(); ++q) tmp[q] = (float)fact[q];
+        HIP_TRY(hipMemcpy2DAsync(P->fact, P->px * 4, tmp.data(), nx * 4, nx * 4, rows, hipMemcpyHostToDevice,
+                                 P->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    P->model_set = true;
+    return AWCPML_OK;
+}
+
+int awcpml_set_cpml(awcpml_plan* P, int axis, const double* a_l, const double* a_r, const double* b_K_l,
+                    const double* b_K_r, const double* a_hl, const double* a_hr, const double* b_K_hl,
+                    const double* b_K_hr) {
+    g_err.clear();
+    int st = check_plan(P);
+    if (st) return st;
+    if (axis < 0 || axis >= P->ndim) return fail(AWCPML_ERR_ARG, "axis %d out of range", axis);
+    const double* v[8] = {a_l, a_r, b_K_l, b_K_r, a_hl, a_hr, b_K_hl, b_K_hr};
+    const int h = P->halo;
+    for (int c = 0; c < 8; ++c) {
+        const size_t len = (size_t)h + (c >= 4 ? 1 : 0);
+        if (len && !v[c]) return fail(AWCPML_ERR_ARG, "coefficient vector %d is NULL", c);
+        if ((st = upload(P, P->coef[axis][c], v[c], len))) return st;
+    }
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    for (int s = 0; s < 2; ++s) {
+        bool idle = true;
+        for (int q = 0; q < h && idle; ++q) idle = v[0 + s][q] == 0.0 && v[2 + s][q] == 1.0;
+        for (int q = 0; q < h + 1 && idle; ++q) idle = v[4 + s][q] == 0.0 && v[6 + s][q] == 1.0;
+        P->active[axis][s] = (h > 0 && !idle) ? 1 : 0;
+    }
+    P->cpml_set[axis] = true;
+    return AWCPML_OK;
+}
+
+int awcpml_set_sources(awcpml_plan* P, int shot, int nsrcs, const int64_t* pos, int64_t nt,
+                       const double* dt2srctf) {
+    g_err.clear();
+    int st = check_plan(P);
+    if (st) return st;
+    if (shot < 0 || shot >= P->nshots) return fail(AWCPML_ERR_ARG, "shot %d out of range", shot);
+    if (nsrcs < 0 || nt < 0) return fail(AWCPML_ERR_ARG, "negative count");
+    if (nsrcs > 0 && (!pos || (nt > 0 && !dt2srctf))) return fail(AWCPML_ERR_ARG, "NULL argument");
+    if (P->nt >= 0 && nt != P->nt) {
+        bool other = false;
+        for (int s = 0; s < P->nshots; ++s) other |= (s != shot && P->src_set[s]);
+        if (other) return fail(AWCPML_ERR_ARG, "nt %lld differs from the plan's nt %lld", (long long)nt, (long long)P->nt);
+    }
+    if ((st = check_positions(P, pos, nsrcs, "source"))) return st;
+    if (nt != P->nt) {
+        // traces of other shots are sized by nt: they are rebuilt on upload
+        P->nt = nt;
+    }
+    P->src_lin[shot].resize(nsrcs);
+    for (int s = 0; s < nsrcs; ++s) P->src_lin[shot][s] = lin_index(P, pos, nsrcs, s);
+    P->src_wav[shot].assign(dt2srctf, dt2srctf + (size_t)nt * nsrcs);
+    P->src_set[shot] = true;
+    P->io_dirty = true;
+    return AWCPML_OK;
+}
+
+int awcpml_set_receivers(awcpml_plan* P, int shot, int nrecs, const int64_t* pos) {
+    g_err.clear();
+    int st = check_plan(P);
+    if (st) return st;
+    if (shot < 0 || shot >= P->nshots) return fail(AWCPML_ERR_ARG, "shot %d out of range", shot);
+    if (nrecs < 0) return fail(AWCPML_ERR_ARG, "negative count");
+    if (nrecs > 0 && !pos) return fail(AWCPML_ERR_ARG, "NULL argument");
+    if ((st = check_positions(P, pos, nrecs, "receiver"))) return st;
+    P->rec_lin[shot].resize(nrecs);
+    for (int r = 0; r < nrecs; ++r) P->rec_lin[shot][r] = lin_index(P, pos, nrecs, r);
+    P->rec_set[shot] = true;
+    P->io_dirty = true;
+    return AWCPML_OK;
+}
+
+int awcpml_reset(awcpml_plan* P) {
+    int st = check_plan(P);
+    if (st) return st;
+    const size_t fbytes = (size_t)P->field_elems * P->nshots * P->esize;
+    for (int b = 0; b < 3; ++b) HIP_TRY(hipMemsetAsync(P->p[b], 0, fbytes, P->stream));
+    for (int a = 0; a < P->ndim; ++a)
+        for (int s = 0; s < 2; ++s) {
+            HIP_TRY(hipMemsetAsync(P->psi[a][s][0], 0, (size_t)P->psi_elems[a] * P->nshots * P->esize, P->stream));
+            HIP_TRY(hipMemsetAsync(P->psi[a][s][1], 0, (size_t)P->psi_elems[a] * P->nshots * P->esize, P->stream));
+            if (P->xi_elems[a])
+                HIP_TRY(hipMemsetAsync(P->xi[a][s], 0, (size_t)P->xi_elems[a] * P->nshots * P->esize, P->stream));
+        }
+    if (P->d_traces && P->n_rec_total > 0 && P->nt > 0)
+        HIP_TRY(hipMemsetAsync(P->d_traces, 0, (size_t)P->n_rec_total * P->nt * P->esize, P->stream));
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    P->it = 0;
+    P->rot = 0;
+    P->par = 0;
+    return AWCPML_OK;
+}
+
+int awcpml_advance(awcpml_plan* P, int64_t nsteps) {
+    g_err.clear();
+    int st = check_plan(P);
+    if (st) return st;
+    if (nsteps < 0) return fail(AWCPML_ERR_ARG, "nsteps < 0");
+    if ((st = ready(P))) return st;
+    if (P->it + nsteps > P->nt)
+        return fail(AWCPML_ERR_STATE, "advance past nt: %lld + %lld > %lld", (long long)P->it,
+                    (long long)nsteps, (long long)P->nt);
+    for (int64_t s = 0; s < nsteps; ++s) {
+        if ((st = do_step(P, P->it))) return st;
+        ++P->it;
+    }
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    return AWCPML_OK;
+}
+
+int awcpml_forward(awcpml_plan* P, int64_t it) {
+    g_err.clear();
+    int st = check_plan(P);
+    if (st) return st;
+    if ((st = ready(P))) return st;
+    if (it < 1 || it > P->nt) return fail(AWCPML_ERR_ARG, "it %lld outside 1..nt", (long long)it);
+    if ((st = do_step(P, it - 1))) return st;
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    return AWCPML_OK;
+}
+
+int awcpml_steps_done(const awcpml_plan* P, int64_t* out) {
+    if (!P || !out) return fail(AWCPML_ERR_ARG, "NULL argument");
+    *out = P->it;
+    return AWCPML_OK;
+}
+
+int awcpml_get_traces(awcpml_plan* P, int shot, double* out) {
+    g_err.clear();
+    int st = check_plan(P);
+    if (st) return st;
+    if (shot < 0 || shot >= P->nshots) return fail(AWCPML_ERR_ARG, "shot %d out of range", shot);
+    if ((st = sync_io(P))) return st;
+    const int64_t nrec = (int64_t)P->rec_lin[shot].size();
+    if (nrec == 0 || P->nt <= 0) return AWCPML_OK;
+    if (!out) return fail(AWCPML_ERR_ARG, "out is NULL");
+    int64_t first = 0;
+    for (int s = 0; s < shot; ++s) first += (int64_t)P->rec_lin[s].size();
+    const size_t count = (size_t)nrec * P->nt;
+    if (P->dtype == AWCPML_F64) {
+        HIP_TRY(hipMemcpyAsync(out, (char*)P->d_traces + (size_t)first * P->nt * 8, count * 8,
+                               hipMemcpyDeviceToHost, P->stream));
+        HIP_TRY(hipStreamSynchronize(P->stream));
+    } else {
+        std::vector<float> tmp(count);
+        HIP_TRY(hipMemcpyAsync(tmp.data(), (char*)P->d_traces + (size_t)first * P->nt * 4, count * 4,
+                               hipMemcpyDeviceToHost, P->stream));
+        HIP_TRY(hipStreamSynchronize(P->stream));
+        for (size_t q = 0; q < count; ++q) out[q] = tmp[q];
+    }
+    return AWCPML_OK;
+}
+
+int awcpml_get_field(awcpml_plan* P, int shot, int which, double* out) {
+    g_err.clear();
+    int st = check_plan(P);
+    if (st) return st;
+    if (shot < 0 || shot >= P->nshots) return fail(AWCPML_ERR_ARG, "shot %d out of range", shot);
+    if (which < 0 || which > 2) return fail(AWCPML_ERR_ARG, "which must be 0, 1 or 2");
+    if (!out) return fail(AWCPML_ERR_ARG, "out is NULL");
+    const char* src = (const char*)P->p[(P->rot + which) % 3] + (size_t)shot * P->field_elems * P->esize;
+    const int64_t nx = P->n[0], rows = P->n[1] * P->n[2];
+    if (P->dtype == AWCPML_F64) {
+        HIP_TRY(hipMemcpy2DAsync(out, nx * 8, src, P->px * 8, nx * 8, rows, hipMemcpyDeviceToHost, P->stream));
+        HIP_TRY(hipStreamSynchronize(P->stream));
+    } else {
+        std::vector<float> tmp((size_t)nx * rows);
+        HIP_TRY(hipMemcpy2DAsync(tmp.data(), nx * 4, src, P->px * 4, nx * 4, rows, hipMemcpyDeviceToHost,
+                                 P->stream));
+        HIP_TRY(hipStreamSynchronize(P->stream));
+        for (size_t q = 0; q < tmp.size(); ++q) out[q] = tmp[q];
+    }
+    return AWCPML_OK;
+}
+
+int awcpml_run(awcpml_plan* P, double* traces, double* pend) {
+    int st = check_plan(P);
+    if (st) return st;
+    if (P->nshots != 1) return fail(AWCPML_ERR_ARG, "awcpml_run needs a one-shot plan");
+    if ((st = awcpml_reset(P))) return st;
+    if ((st = awcpml_advance(P, P->nt))) return st;
+    if (traces && (st = awcpml_get_traces(P, 0, traces))) return st;
+    if (pend && (st = awcpml_get_field(P, 0, 1, pend))) return st;
+    return AWCPML_OK;
+}
+
+int awcpml_time_stencil(awcpml_plan* P, int nlaunches, double* ms_total) {
+    g_err.clear();
+    int st = check_plan(P);
+    if (st) return st;
+    if (!ms_total || nlaunches < 1) return fail(AWCPML_ERR_ARG, "bad argument");
+    if ((st = ready(P))) return st;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, P->stream));
+    for (int q = 0; q < nlaunches; ++q) {
+        if ((st = launch_stencil(P))) break;
+        P->rot = (P->rot + 1) % 3;
+        P->par ^= 1;
+    }
+    HIP_TRY(hipEventRecord(e1, P->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_total = ms;
+    return st;
+}
+
+int awcpml_time_advance(awcpml_plan* P, int64_t nsteps, double* ms_total) {
+    g_err.clear();
+    int st = check_plan(P);
+    if (st) return st;
+    if (!ms_total || nsteps < 0) return fail(AWCPML_ERR_ARG, "bad argument");
+    if ((st = ready(P))) return st;
+    if (P->it + nsteps > P->nt) return fail(AWCPML_ERR_STATE, "advance past nt");
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, P->stream));
+    for (int64_t s = 0; s < nsteps; ++s) {
+        if ((st = do_step(P, P->it))) break;
+        ++P->it;
+    }
+    HIP_TRY(hipEventRecord(e1, P->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_total = ms;
+    return st;
+}
+
+int awcpml_set_tiling(awcpml_plan* P, int block_y, int march_len) {
+    g_err.clear();
+    int st = check_plan(P);
+    if (st) return st;
+    if (block_y != 0 && block_y != 1 && block_y != 2 && block_y != 4)
+        return fail(AWCPML_ERR_ARG, "block_y must be 1, 2 or 4 (got %d)", block_y);
+    if (march_len < 0) return fail(AWCPML_ERR_ARG, "march_len < 0");
+    P->block_y = block_y;
+    P->march_len = march_len;
+    return default_tiling(P);
+}
+
+int awcpml_get_tiling(const awcpml_plan* P, int* block_y, int* march_len) {
+    if (!P || !block_y || !march_len) return fail(AWCPML_ERR_ARG, "NULL argument");
+    *block_y = P->block_y;
+    *march_len = P->march_len;
+    return AWCPML_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,218 @@
+"""Throughput of the CPML time-stepping hot path on one MI355X.
+
+Workload: the 3D plain CPML forward solve -- 256^3 interior + 20-point CPML
+halo (298^3 nodes), free top, dx = 10 m, c(depth) = 1500 + 0.6 depth with a
++-5 % seeded smoothed perturbation, Ricker f0 = 10 Hz, 64 receivers,
+nt = 2000 time steps (tests/cases.py: plain3d).
+
+One bench "step" is one whole solve of that workload: zero the state and run
+all nt time steps of every shot (inputs already resident in HBM).  Three
+variants run on the same model:
+  fp64_plain    FP64, one shot per launch (the drop-in path)
+  fp32          FP32, one shot
+  fp32_batched  FP32, S shots per launch (fact and CPML work shared)
+`value` is the FP32-batched throughput in shot*Gpts/s (shot x nodes x nt /
+time); the FP64 plain figure and the ratio are reported beside it.
+
+The roofline entry is for the dominant kernel (the fused stencil) of the
+`value` variant: algorithmic bytes per launch
+  B = sizeof(T) * [S*(3N + C) + N],  C = sum over absorbing sides of
+      (4*halo + 2) * N / n_axis
+divided by its average launch time from HIP events on the plan's stream.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "acousticwavecpml.jl_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+METRIC = ("shot·Gpts/s + effective HBM GB/s (% of peak), FP32-batched vs FP64-plain, "
+          "same GPU")
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(n, halo, freetop, esize, nshots):
+    N = 1
+    for v in n:
+        N *= v
+    C = 0
+    for a, na in enumerate(n):
+        sides = 1 if (freetop and a == 1) else 2
+        if halo > 0:
+            C += sides * (4 * halo + 2) * (N // na)
+    return esize * (nshots * (3 * N + C) + N), N
+
+
+def build_plan(S, dtype, nshots):
+    import awcpml
+    P = awcpml.Plan(S.n, S.halo, dtype=dtype, nshots=nshots)
+    P.set_model(S.fact, S.inv_d, S.inv_d2)
+    for a in range(S.ndim):
+        P.set_cpml(a, S.coef[a])
+    for s in range(nshots):
+        # shots spread along x at the source depth, like a line survey
+        pos = S.possrcs.copy()
+        pos[0, 0] = 1 + S.halo + 1 + (s * (S.n[0] - 2 * S.halo - 4)) // max(nshots, 1)
+        if nshots == 1:
+            pos = S.possrcs
+        P.set_sources(s, pos, S.dt2srctf)
+        P.set_receivers(s, S.posrecs)
+    return P
+
+
+def time_variant(S, dtype, nshots, steps, warmup, freetop, kernel_launches=50):
+    P = build_plan(S, dtype, nshots)
+    esize = 8 if dtype == "f64" else 4
+    for _ in range(warmup):
+        P.reset()
+        P.advance(S.nt)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        P.reset()
+        P.advance(S.nt)
+    t = (time.perf_counter() - t0) / max(steps, 1)
+    traces0 = P.traces(0)
+    # dominant kernel alone, HIP events on the plan's stream
+    P.reset()
+    ms = P.time_stencil(kernel_launches) / kernel_launches
+    tiling = P.tiling()
+    P.close()
+    B, N = algorithmic_bytes(S.n, S.halo, freetop, esize, nshots)
+    pts = nshots * N * S.nt
+    return {
+        "dtype": dtype, "shots": nshots, "s_per_solve": t,
+        "shot_gpts_per_s": pts / t / 1e9,
+        "effective_gbs": B * S.nt / t / 1e9,
+        "kernel_ms": ms, "kernel_gbs": B / (ms * 1e-3) / 1e9,
+        "bytes_per_launch": B, "tiling": {"block_y": tiling[0], "march_len": tiling[1]},
+        "_traces0": traces0,
+    }
+
+
+def cpu_baseline(case, max_seconds=20.0):
+    """Oracle (C FP64 restatement of the reference's Threads path) on the
+    same grid for a bounded number of time steps."""
+    import oracle
+    o = oracle.OracleProblem(case["n"], case["l"], case["lt"], case["vel"], case["srcpos"],
+                             case["t0s"], case["f0"], case["recpos"], halo=case["halo"],
+                             rcoef=case["rcoef"], ppw=case["ppw"], freetop=case["freetop"])
+    N = 1
+    for v in case["n"]:
+        N *= v
+    t0 = time.perf_counter()
+    o.run(2)
+    per = (time.perf_counter() - t0) / 2
+    nsteps = max(2, min(o.nt - 2, int(max_seconds / max(per, 1e-6))))
+    t0 = time.perf_counter()
+    done = o.run(nsteps)
+    t = time.perf_counter() - t0
+    o.close()
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    return {
+        "value": done * N / t / 1e9, "unit": "shot·Gpts/s", "cores": cores, "kind": "port",
+        "sample": f"plain3d {case['n'][0]}^3 FP64 single shot, time steps 3..{2 + done} of "
+                  f"{o.nt} ({t:.1f} s), C/OpenMP oracle",
+    }
+
+
+def load_traffic(kernel, workload):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            rec = json.load(fh)
+        ent = rec.get(kernel, {})
+        if ent.get("workload") == workload:
+            return ent.get("bytes_per_launch")
+    except (OSError, ValueError):
+        pass
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--shots", type=int, default=8)
+    ap.add_argument("--nt", type=int, default=2000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from awcpml.host import setup
+    import awcpml
+    from cases import plain3d, rel_l2
+
+    case = plain3d(nt=args.nt)
+    srcs = awcpml.Sources(case["srcpos"], case["t0s"], [awcpml.rickersource1D], case["f0"])
+    recs = awcpml.Receivers(case["recpos"])
+    S = setup(case["l"], case["lt"], case["vel"], srcs, recs, halo=case["halo"],
+              rcoef=case["rcoef"], ppw=case["ppw"], freetop=case["freetop"])
+    print(f"[bench] plain3d n={S.n} nt={S.nt} halo={S.halo}", file=sys.stderr, flush=True)
+
+    v64 = time_variant(S, "f64", 1, args.steps, args.warmup, case["freetop"])
+    print(f"[bench] fp64 plain: {v64['shot_gpts_per_s']:.2f} shot*Gpts/s", file=sys.stderr, flush=True)
+    v32 = time_variant(S, "f32", 1, args.steps, args.warmup, case["freetop"])
+    print(f"[bench] fp32 1 shot: {v32['shot_gpts_per_s']:.2f} shot*Gpts/s", file=sys.stderr, flush=True)
+    vb = time_variant(S, "f32", args.shots, args.steps, args.warmup, case["freetop"])
+    print(f"[bench] fp32 {args.shots} shots: {vb['shot_gpts_per_s']:.2f} shot*Gpts/s", file=sys.stderr,
+          flush=True)
+    fp32_err = rel_l2(v32["_traces0"], v64["_traces0"])
+
+    main_v = vb
+    kname = "stencil3d<float>"
+    workload = f"plain3d_{S.n[0]}^3_nt{S.nt}_f32_S{args.shots}"
+    achieved = main_v["kernel_gbs"]
+    traffic = load_traffic(kname, workload)
+    out = {
+        "metric": METRIC,
+        "value": round(main_v["shot_gpts_per_s"], 3),
+        "unit": "shot·Gpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(main_v["s_per_solve"] * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain3d: 3D CPML forward solve, {S.n[0]}^3 nodes (256^3 interior + "
+                        f"20-point halo), free top, nt={S.nt}, FP32, {args.shots} shots per launch",
+            "nodes": list(S.n), "nt": S.nt, "halo": S.halo, "shots": args.shots,
+            "effective_gbs": round(main_v["effective_gbs"], 1),
+            "effective_frac_of_peak": round(main_v["effective_gbs"] / HBM_PEAK_GBS, 4),
+            "fp64_plain": {k: (round(v, 4) if isinstance(v, float) else v)
+                           for k, v in v64.items() if not k.startswith("_")},
+            "fp32_single": {k: (round(v, 4) if isinstance(v, float) else v)
+                            for k, v in v32.items() if not k.startswith("_")},
+            "fp32_vs_fp64_plain": round(main_v["shot_gpts_per_s"] / v64["shot_gpts_per_s"], 3),
+            "fp32_trace_rel_l2_vs_fp64": fp32_err,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": kname, "bytes_per_launch": main_v["bytes_per_launch"],
+            "kernel_ms": round(main_v["kernel_ms"], 5),
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        print("[bench] cpu baseline (oracle)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,156 @@
+/*
+ * awcpml.h -- C ABI of the MI355X (gfx950) CPML acoustic time-stepping
+ * library libawcpml.so.
+ *
+ * The library replaces the per-step part of AcousticWaveCPML.jl's xPU
+ * solvers: the kernels update_psi_{x,y,z}!, update_p!, inject_sources!,
+ * record_receivers! and the forward! driver with its (pold, pcur, pnew)
+ * rotation (src/shared/acoustic_2D_xPU.jl:7-131,327-345 and
+ * src/shared/acoustic_3D_xPU.jl:8-167,392-407).  The host-side setup that
+ * the reference does in FP64 before its time loop (CFL dt, calc_Kab_CPML,
+ * fact = dt^2 vel^2, Ricker wavelets, position snapping;
+ * src/shared/acoustic_2D_xPU.jl:151-240) stays with the caller, which hands
+ * its results to the plan below.  That is the seam a Julia
+ * Acoustic{2,3}D_HIP submodule binds with ccall.
+ *
+ * Conventions
+ *  - Every array is a plain host pointer, column-major with x fastest,
+ *    exactly the memory order of the reference's Julia arrays.
+ *  - Grid-node positions are 1-based, as in the reference
+ *    (possrcs/posrecs, src/shared/acoustic_2D_xPU.jl:228-240).
+ *  - All coefficients are passed in FP64; a plan of dtype AWCPML_F32 rounds
+ *    them to FP32 once, on upload.
+ *  - Every call returns an int status: AWCPML_OK (0) or a negative code;
+ *    awcpml_last_error() gives the message of the calling thread's last
+ *    failure.  Nothing aborts the process.
+ *  - A plan owns its device buffers and one HIP stream; calls are
+ *    synchronous for the caller.  A plan must not be used by two threads at
+ *    once; distinct plans are independent.
+ */
+#ifndef AWCPML_H
+#define AWCPML_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AWCPML_OK 0
+#define AWCPML_ERR_ARG -1      /* invalid argument or shape              */
+#define AWCPML_ERR_HIP -2      /* HIP runtime error                      */
+#define AWCPML_ERR_NOMEM -3    /* device allocation failed               */
+#define AWCPML_ERR_STATE -4    /* call out of order (e.g. no sources set) */
+#define AWCPML_ERR_RANGE -5    /* a source/receiver outside the model    */
+
+#define AWCPML_F64 0
+#define AWCPML_F32 1
+
+typedef struct awcpml_plan awcpml_plan;
+
+/* Create a plan for `nshots` independent shots on one velocity model.
+ * ndim 2 (nz ignored) or 3; n >= 3 per axis and n >= 2*halo+2 per axis
+ * (the two CPML layers of an axis must not overlap); halo >= 0.
+ * dtype AWCPML_F64 reproduces the reference's Float64 arithmetic operation
+ * for operation; AWCPML_F32 keeps state and arithmetic in FP32.
+ * Replaces the array allocation of src/shared/acoustic_2D_xPU.jl:208-215
+ * (3D: src/shared/acoustic_3D_xPU.jl:256-267). */
+int awcpml_plan_create(int ndim, int64_t nx, int64_t ny, int64_t nz, int halo,
+                       int dtype, int nshots, awcpml_plan **out);
+
+int awcpml_plan_destroy(awcpml_plan *plan);
+
+/* Model: fact = dt^2 .* vel.^2 (prod(n) values) and the inverse spacings
+ * _dx,_dy(,_dz) and _dx2,_dy2(,_dz2)
+ * (src/shared/acoustic_2D_xPU.jl:190-196; 3D: acoustic_3D_xPU.jl:236-245).
+ * inv_d and inv_d2 hold ndim values each. */
+int awcpml_set_model(awcpml_plan *plan, const double *fact,
+                     const double *inv_d, const double *inv_d2);
+
+/* CPML coefficients of one axis (0 = x, 1 = y, 2 = z), as returned by
+ * calc_Kab_CPML "ongrd" (a_l, a_r, b_K_l, b_K_r: halo values) and
+ * "halfgrd" (a_hl, a_hr, b_K_hl, b_K_hr: halo+1 values), after the free-top
+ * override (src/utils.jl:19-57; src/shared/acoustic_2D_xPU.jl:175-185).
+ * A side whose a's are all 0 and b_K's all 1 (the free top) is detected
+ * and skipped by the kernels: its psi/xi stay exactly zero, as in the
+ * reference. */
+int awcpml_set_cpml(awcpml_plan *plan, int axis,
+                    const double *a_l, const double *a_r,
+                    const double *b_K_l, const double *b_K_r,
+                    const double *a_hl, const double *a_hr,
+                    const double *b_K_hl, const double *b_K_hr);
+
+/* Sources of one shot: pos is nsrcs x ndim (column-major, 1-based grid
+ * indices, possrcs of src/shared/acoustic_2D_xPU.jl:228-232), dt2srctf is
+ * nt x nsrcs column-major (acoustic_2D_xPU.jl:221-226).  nt is the number
+ * of time steps of the run and must be the same for all shots of a plan.
+ * Sources at the same node are added in source order, like the reference's
+ * plain-loop twin (src/shared/acoustic_2D.jl:81-88). */
+int awcpml_set_sources(awcpml_plan *plan, int shot, int nsrcs,
+                       const int64_t *pos, int64_t nt, const double *dt2srctf);
+
+/* Receivers of one shot: pos is nrecs x ndim, 1-based
+ * (posrecs, src/shared/acoustic_2D_xPU.jl:233-240). */
+int awcpml_set_receivers(awcpml_plan *plan, int shot, int nrecs,
+                         const int64_t *pos);
+
+/* Zero the wavefields, psi/xi and traces and set the step counter to 0
+ * (the @zeros state of acoustic_2D_xPU.jl:208-215). */
+int awcpml_reset(awcpml_plan *plan);
+
+/* Run time steps it+1 .. it+nsteps of every shot (the `for it=1:nt` loop of
+ * src/shared/acoustic_2D_xPU.jl:327-345 / acoustic_3D_xPU.jl:392-407,
+ * including receiver recording).  Fails with AWCPML_ERR_STATE past nt. */
+int awcpml_advance(awcpml_plan *plan, int64_t nsteps);
+
+/* One forward! call for an explicit step index `it` (1-based), including
+ * the buffer rotation; the do_bench path of acoustic_2D_xPU.jl:261-271
+ * times this with it = 1.  Does not move the step counter. */
+int awcpml_forward(awcpml_plan *plan, int64_t it);
+
+/* Number of time steps completed since the last reset. */
+int awcpml_steps_done(const awcpml_plan *plan, int64_t *out);
+
+/* Receiver traces of one shot, nt x nrecs column-major in FP64
+ * (recs.seismograms = Array(traces), acoustic_2D_xPU.jl:441). */
+int awcpml_get_traces(awcpml_plan *plan, int shot, double *out);
+
+/* Copy one wavefield of a shot to the host in FP64, prod(n) values.
+ * which = 1 is pcur, the field solve2D/solve3D returns
+ * (acoustic_2D_xPU.jl:444) and the 3D do_save snapshot writes
+ * (acoustic_3D_xPU.jl:480-498); 0 = pold, 2 = pnew. */
+int awcpml_get_field(awcpml_plan *plan, int shot, int which, double *out);
+
+/* Convenience for one-shot plans: reset, run nt steps, copy the traces
+ * (nt x nrecs) and the final field; either output pointer may be NULL. */
+int awcpml_run(awcpml_plan *plan, double *traces, double *pend);
+
+/* Time `nlaunches` launches of the plan's stencil kernel alone (no source
+ * or receiver work) between two HIP events on the plan's stream and return
+ * the total in milliseconds.  The wavefield state advances without
+ * sources; call awcpml_reset afterwards.  Measurement support, no
+ * reference counterpart (the reference times whole forward! calls with
+ * BenchmarkTools, acoustic_2D_xPU.jl:263-276). */
+int awcpml_time_stencil(awcpml_plan *plan, int nlaunches, double *ms_total);
+
+/* Time `nsteps` full steps (awcpml_advance) between two HIP events on the
+ * plan's stream, in milliseconds.  Measurement support (see above). */
+int awcpml_time_advance(awcpml_plan *plan, int64_t nsteps, double *ms_total);
+
+/* Launch geometry of the stencil kernel: threads per block along y
+ * (1, 2 or 4) and the length of the z (3D) or y (2D) march per block.
+ * 0 keeps the library's choice.  Used by the autotuner. */
+int awcpml_set_tiling(awcpml_plan *plan, int block_y, int march_len);
+int awcpml_get_tiling(const awcpml_plan *plan, int *block_y, int *march_len);
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char *awcpml_last_error(void);
+
+/* Library version string. */
+const char *awcpml_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AWCPML_H */

@@ -1,0 +1,155 @@
+"""Problem definitions shared by the tests, the smoke check and bench.py.
+
+Each case is a plain dict of the reference's solve2D/solve3D arguments
+(positions in meters, rows = items).  The workloads follow the reference's
+own tests (test/test2D_xPU.jl, test/test3D_xPU.jl) and the benchmark
+configurations:
+
+  plain2d   2D CPML, 256x256 interior + 20-point halo on all sides
+            (298x298 nodes), dx = 10 m, c = 2000 m/s, R = 1e-3, f0 = 8 Hz,
+            source at the centre, 8 receivers 20 points below the top
+            halo, nt = 1000.
+  plain3d   3D CPML, 256^3 interior + 20-point halo, free top, dx = 10 m,
+            c(depth) = 1500 + 0.6 depth, +-5 % seeded smoothed perturbation
+            (sigma = 5 points), f0 = 10 Hz, source at 0-based (149, 30, 149),
+            64 receivers on an 8x8 grid 5 nodes below the surface, nt = 2000.
+
+Depth is the reference's second index (y); the free top is the y-low side.
+n = interior + 2*halo + 2 per axis, lengths = (n-1)*dx, and
+lt = (nt - 0.5)*dt reproduces nt = ceil(lt/dt) exactly.
+"""
+import json
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+
+
+def cfl_dt(ndim, d, vmax):
+    if ndim == 2:
+        return math.sqrt(2) / (vmax * (1 / d[0] + 1 / d[1])) / 2
+    return math.sqrt(3) / (vmax * (1 / d[0] + 1 / d[1] + 1 / d[2])) / 2
+
+
+def lt_for_nt(ndim, d, vmax, nt):
+    return (nt - 0.5) * cfl_dt(ndim, d, vmax)
+
+
+def interior_grid(interior, halo, count):
+    """0-based node indices of `count` points evenly spread over the interior."""
+    first = halo + 1
+    return [first + int((k + 0.5) * interior / count) for k in range(count)]
+
+
+def load_fixtures():
+    with open(os.path.join(GOLDEN, "fixtures.json")) as fh:
+        meta = json.load(fh)
+    out = {}
+    for name, m in meta.items():
+        ref = np.fromfile(os.path.join(GOLDEN, m["file"]), dtype="<f8").reshape(m["shape"], order="F")
+        case = fixture_case(m)
+        case["name"] = name
+        out[name] = (case, ref)
+    return out
+
+
+def fixture_case(m):
+    n = m["n"]
+    if m["vel"]["kind"] == "constant":
+        vel = np.full(n, m["vel"]["c"], order="F")
+    else:  # test/test2D_xPU.jl:160-169
+        vel = np.zeros(n, order="F")
+        for j in range(n[1]):
+            vel[:, j] = m["vel"]["v0"] + m["vel"]["dv"] * j
+        jc = n[1] - 1 - m["vel"]["const_from_end"]
+        vel[:, jc:] = vel[0, jc]
+    return dict(n=n, l=m["l"], lt=m["lt"], vel=vel, srcpos=np.array(m["srcpos"]),
+                t0s=[m["t0"]] * len(m["srcpos"]), f0=m["f0"], recpos=np.array(m["recpos"]),
+                halo=m["halo"], rcoef=m["rcoef"], freetop=m["freetop"], ppw=10.0)
+
+
+def plain2d(nt=1000, interior=256, halo=20, dx=10.0, c=2000.0):
+    n = interior + 2 * halo + 2
+    d = (dx, dx)
+    L = (n - 1) * dx
+    rec_i = interior_grid(interior, halo, 8)
+    rec_j = halo + 1 + 20
+    return dict(n=[n, n], l=[L, L], lt=lt_for_nt(2, d, c, nt), vel=np.full((n, n), c, order="F"),
+                srcpos=np.array([[L / 2, L / 2]]), t0s=[4 / 8.0], f0=8.0,
+                recpos=np.array([[i * dx, rec_j * dx] for i in rec_i]),
+                halo=halo, rcoef=1e-3, freetop=False, ppw=10.0, nt=nt)
+
+
+def smooth_random(shape, sigma, seed):
+    """Seeded N(0,1) field, Gaussian-smoothed (sigma in points), max|g| = 1."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.Generator(np.random.PCG64(seed))
+    g = rng.standard_normal(shape)
+    g = gaussian_filter(g, sigma=sigma, mode="nearest")
+    return g / np.abs(g).max()
+
+
+def plain3d(nt=2000, interior=256, halo=20, dx=10.0, seed=2024, nrec_side=8):
+    n = interior + 2 * halo + 2
+    d = (dx, dx, dx)
+    L = (n - 1) * dx
+    depth = np.arange(n) * dx
+    base = (1500.0 + 0.6 * depth)[None, :, None]
+    vel = np.asfortranarray(base * (1.0 + 0.05 * smooth_random((n, n, n), 5.0, seed)))
+    vmax = float(vel.max())
+    src = [(halo + 1 + interior // 2) * dx, 30 * dx, (halo + 1 + interior // 2) * dx]
+    grid = interior_grid(interior, halo, nrec_side)
+    rec = [[i * dx, 5 * dx, k * dx] for k in grid for i in grid]
+    return dict(n=[n, n, n], l=[L, L, L], lt=lt_for_nt(3, d, vmax, nt), vel=vel,
+                srcpos=np.array([src]), t0s=[4 / 10.0], f0=10.0, recpos=np.array(rec),
+                halo=halo, rcoef=1e-3, freetop=True, ppw=10.0, nt=nt)
+
+
+def small3d(n=(37, 29, 23), halo=5, freetop=True, nt=120, seed=7, nsrc=1):
+    """Tiny heterogeneous 3D case for exact comparisons."""
+    dx = 10.0
+    d = (dx, dx, dx)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    vel = np.asfortranarray(1800.0 + 400.0 * rng.random(n))
+    vmax = float(vel.max())
+    L = [(v - 1) * dx for v in n]
+    srcs = [[(n[0] // 2) * dx, (halo + 3) * dx, (n[2] // 2) * dx]]
+    for s in range(1, nsrc):
+        srcs.append([((n[0] // 2) + s) * dx, (halo + 3) * dx, (n[2] // 3) * dx])
+    recs = [[(halo + 2 + 3 * r) * dx, (halo + 2) * dx, (n[2] // 2 + r - 2) * dx] for r in range(5)]
+    recs.append([0.0, 0.0, 0.0])  # a Dirichlet corner node
+    return dict(n=list(n), l=L, lt=lt_for_nt(3, d, vmax, nt), vel=vel, srcpos=np.array(srcs),
+                t0s=[0.02] * len(srcs), f0=25.0, recpos=np.array(recs), halo=halo, rcoef=1e-4,
+                freetop=freetop, ppw=2.0, nt=nt)
+
+
+def small2d(n=(57, 43), halo=6, freetop=True, nt=300, seed=3):
+    dx = 10.0
+    d = (dx, dx)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    vel = np.asfortranarray(1800.0 + 400.0 * rng.random(n))
+    vmax = float(vel.max())
+    L = [(v - 1) * dx for v in n]
+    srcs = [[(n[0] // 2) * dx, (halo + 4) * dx], [(n[0] // 3) * dx, 0.0]]  # one on the top row
+    recs = [[(halo + 2 + 4 * r) * dx, (halo + 2) * dx] for r in range(6)] + [[(n[0] // 3) * dx, 0.0]]
+    return dict(n=list(n), l=L, lt=lt_for_nt(2, d, vmax, nt), vel=vel, srcpos=np.array(srcs),
+                t0s=[0.03, 0.05], f0=20.0, recpos=np.array(recs), halo=halo, rcoef=1e-4,
+                freetop=freetop, ppw=2.0, nt=nt)
+
+
+def oracle_problem(case):
+    import oracle
+    return oracle.OracleProblem(case["n"], case["l"], case["lt"], case["vel"], case["srcpos"],
+                                case["t0s"], case["f0"], case["recpos"], halo=case["halo"],
+                                rcoef=case["rcoef"], ppw=case.get("ppw", 10.0),
+                                freetop=case["freetop"])
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    den = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / den) if den > 0 else float(np.linalg.norm(a - b))

@@ -1,0 +1,134 @@
+"""GPU parity of the HIP path against the CPU FP64 oracle and the
+reference's golden fields.
+
+FP64 bar: bit-for-bit equality of traces and final field when the GPU plan is
+fed the oracle's own setup arrays (same coefficients, wavelets, positions);
+<= 1e-12 relative L2 when the product's host setup (numpy) is used end to
+end.  FP32 bar: receiver traces within 1e-5 relative L2 of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+import awcpml
+from cases import (fixture_case, load_fixtures, oracle_problem, plain2d, rel_l2, small2d,
+                   small3d)
+
+pytestmark = pytest.mark.gpu
+
+FP64_E2E_TOL = 1e-12   # end to end, product host setup vs oracle setup
+FP32_TRACE_TOL = 1e-5  # FP32 receiver traces vs the FP64 oracle
+SQRT_EPS = 1.4901161193847656e-08  # the reference's isapprox default
+
+
+def plan_from_oracle(o, dtype="f64", nshots=1):
+    P = awcpml.Plan(o.n, o.halo, dtype=dtype, nshots=nshots)
+    inv_d = [1.0 / d for d in _spacing(o)]
+    inv_d2 = [1.0 / (d * d) for d in _spacing(o)]
+    P.set_model(o.fact(), inv_d, inv_d2)
+    for a in range(o.ndim):
+        P.set_cpml(a, [o.coef(a, w) for w in range(8)])
+    ps, pr = o.positions()
+    for s in range(nshots):
+        P.set_sources(s, ps, o.dt2srctf())
+        P.set_receivers(s, pr)
+    return P
+
+
+def _spacing(o):
+    return o._spacing
+
+
+def run_oracle(case, nsteps=None):
+    o = oracle_problem(case)
+    o._spacing = [case["l"][a] / (case["n"][a] - 1) for a in range(len(case["n"]))]
+    o.run(nsteps)
+    return o
+
+
+def run_gpu(o, nsteps=None, dtype="f64"):
+    P = plan_from_oracle(o, dtype=dtype)
+    P.reset()
+    P.advance(o.it_done if nsteps is None else nsteps)
+    tr, pend = P.traces(0), P.field(0, 1)
+    P.close()
+    return tr, pend
+
+
+def solve_case(case, dtype="f64"):
+    srcs = awcpml.Sources(case["srcpos"], case["t0s"], [awcpml.rickersource1D] * len(case["t0s"]),
+                          case["f0"])
+    recs = awcpml.Receivers(case["recpos"])
+    kw = dict(halo=case["halo"], rcoef=case["rcoef"], ppw=case.get("ppw", 10.0),
+              freetop=case["freetop"], dtype=dtype)
+    if len(case["n"]) == 2:
+        pend = awcpml.solve2D(case["l"][0], case["l"][1], case["lt"], case["vel"], srcs, recs, **kw)
+    else:
+        pend = awcpml.solve3D(case["l"][0], case["l"][1], case["l"][2], case["lt"], case["vel"],
+                              srcs, recs, **kw)
+    return pend, recs.seismograms
+
+
+@pytest.mark.parametrize("name", ["acoustic2D_center_halo20", "acoustic2D_center_freetop_halo20",
+                                  "acoustic2D_gradient_freetop_halo20"])
+def test_golden_fields_2d(name):
+    """The reference's three golden fields (test/test2D_xPU.jl:108-195)."""
+    case, ref = load_fixtures()[name]
+    print(f"[gpu] golden {name} n={case['n']}", flush=True)
+    pend, traces = solve_case(case)
+    o = run_oracle(case)
+    assert rel_l2(pend, ref) <= SQRT_EPS
+    assert rel_l2(pend, o.pend()) <= FP64_E2E_TOL
+    assert rel_l2(traces, o.traces()) <= FP64_E2E_TOL
+    # kernel bitwise: same setup arrays as the oracle
+    tr_b, pend_b = run_gpu(o)
+    assert np.array_equal(pend_b, o.pend())
+    assert np.array_equal(tr_b, o.traces())
+
+
+@pytest.mark.parametrize("freetop", [False, True])
+def test_small2d_bitwise(freetop):
+    case = small2d(freetop=freetop)
+    print(f"[gpu] small2d freetop={freetop}", flush=True)
+    o = run_oracle(case)
+    tr, pend = run_gpu(o)
+    assert np.array_equal(tr, o.traces())
+    assert np.array_equal(pend, o.pend())
+
+
+@pytest.mark.parametrize("freetop,nsrc", [(True, 1), (False, 2)])
+def test_small3d_bitwise(freetop, nsrc):
+    case = small3d(freetop=freetop, nsrc=nsrc)
+    print(f"[gpu] small3d freetop={freetop} nsrc={nsrc}", flush=True)
+    o = run_oracle(case)
+    tr, pend = run_gpu(o)
+    assert np.array_equal(tr, o.traces())
+    assert np.array_equal(pend, o.pend())
+
+
+def test_plain2d_config_traces():
+    case = plain2d()
+    print("[gpu] plain2d 298^2 nt=1000", flush=True)
+    o = run_oracle(case)
+    assert o.nt == 1000
+    pend, traces = solve_case(case)
+    assert rel_l2(traces, o.traces()) <= FP64_E2E_TOL
+    assert rel_l2(pend, o.pend()) <= FP64_E2E_TOL
+    tr_b, pend_b = run_gpu(o)
+    assert np.array_equal(tr_b, o.traces())
+    assert np.array_equal(pend_b, o.pend())
+
+
+def test_fp32_traces_2d():
+    case = plain2d()
+    print("[gpu] plain2d fp32", flush=True)
+    o = run_oracle(case)
+    _, traces = solve_case(case, dtype="f32")
+    assert rel_l2(traces, o.traces()) <= FP32_TRACE_TOL
+
+
+def test_fp32_traces_3d_small():
+    case = small3d(n=(61, 53, 47), halo=8, nt=300)
+    print("[gpu] small3d fp32", flush=True)
+    o = run_oracle(case)
+    tr, _ = run_gpu(o, dtype="f32")
+    assert rel_l2(tr, o.traces()) <= FP32_TRACE_TOL
