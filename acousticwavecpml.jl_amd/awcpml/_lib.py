@@ -44,8 +44,9 @@ SIGNATURES = {
     "awcpml_run": (ctypes.c_int, [_vp, _dp, _dp]),
     "awcpml_time_stencil": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "awcpml_time_advance": (ctypes.c_int, [_vp, ctypes.c_int64, _dp]),
-    "awcpml_set_tiling": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "awcpml_set_tiling": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "awcpml_get_tiling": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int),
+                                         ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_int)]),
     "awcpml_last_error": (ctypes.c_char_p, []),
     "awcpml_version": (ctypes.c_char_p, []),

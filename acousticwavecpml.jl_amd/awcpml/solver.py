@@ -98,13 +98,15 @@ class Plan:
             self._nrecs = {}
         self._nrecs[shot] = pos.shape[0]
 
-    def set_tiling(self, block_y=0, march_len=0):
-        _lib.check(_lib.load().awcpml_set_tiling(self._h, block_y, march_len))
+    def set_tiling(self, block_y=0, march_len=0, prefetch=0):
+        _lib.check(_lib.load().awcpml_set_tiling(self._h, block_y, march_len, prefetch))
 
     def tiling(self):
-        by, ml = ctypes.c_int(), ctypes.c_int()
-        _lib.check(_lib.load().awcpml_get_tiling(self._h, ctypes.byref(by), ctypes.byref(ml)))
-        return by.value, ml.value
+        """(block_y, march_len, prefetch) of the stencil launches."""
+        by, ml, pf = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.load().awcpml_get_tiling(self._h, ctypes.byref(by), ctypes.byref(ml),
+                                                 ctypes.byref(pf)))
+        return by.value, ml.value, pf.value
 
     # -- execution ----------------------------------------------------
     def reset(self):

@@ -94,6 +94,7 @@ struct awcpml_plan {
 
     hipStream_t stream = nullptr;
     int block_y = 0, march_len = 0;
+    int prefetch = 2;  // software-pipeline depth of the march (1..3)
     int device = 0;
 };
 
@@ -204,15 +205,36 @@ StencilArgs<T> make_args(const awcpml_plan* P) {
     return A;
 }
 
+template <typename T, typename Off, int D>
+void launch_variant(const awcpml_plan* P, const StencilArgs<T>& A) {
+    dim3 block(awcpml::kTileX, P->block_y);
+    dim3 grid(A.grid_blocks);
+    // march-axis CPML coefficients staged in LDS: 8 vectors of halo+1
+    const size_t lds = (size_t)8 * (P->halo + 1) * sizeof(T);
+    if (P->ndim == 3)
+        awcpml::stencil3d<T, Off, D><<<grid, block, lds, P->stream>>>(A);
+    else
+        awcpml::stencil2d<T, Off, D><<<grid, block, lds, P->stream>>>(A);
+}
+
+template <typename T, typename Off>
+void launch_depth(const awcpml_plan* P, const StencilArgs<T>& A) {
+    switch (P->prefetch) {
+        case 1: launch_variant<T, Off, 1>(P, A); break;
+        case 3: launch_variant<T, Off, 3>(P, A); break;
+        default: launch_variant<T, Off, 2>(P, A); break;
+    }
+}
+
 template <typename T>
 int launch_stencil_t(awcpml_plan* P) {
     StencilArgs<T> A = make_args<T>(P);
-    dim3 block(awcpml::kTileX, P->block_y);
-    dim3 grid(A.grid_blocks);
-    if (P->ndim == 3)
-        awcpml::stencil3d<T><<<grid, block, 0, P->stream>>>(A);
+    // 32-bit byte offsets when one shot's field fits in 4 GiB (the slabs
+    // are smaller than a field)
+    if ((uint64_t)P->field_elems * sizeof(T) < (1ull << 32))
+        launch_depth<T, uint32_t>(P, A);
     else
-        awcpml::stencil2d<T><<<grid, block, 0, P->stream>>>(A);
+        launch_depth<T, uint64_t>(P, A);
     HIP_TRY(hipGetLastError());
     return AWCPML_OK;
 }
@@ -689,22 +711,25 @@ int awcpml_time_advance(awcpml_plan* P, int64_t nsteps, double* ms_total) {
     return st;
 }
 
-int awcpml_set_tiling(awcpml_plan* P, int block_y, int march_len) {
+int awcpml_set_tiling(awcpml_plan* P, int block_y, int march_len, int prefetch) {
     g_err.clear();
     int st = check_plan(P);
     if (st) return st;
     if (block_y != 0 && block_y != 1 && block_y != 2 && block_y != 4)
         return fail(AWCPML_ERR_ARG, "block_y must be 1, 2 or 4 (got %d)", block_y);
     if (march_len < 0) return fail(AWCPML_ERR_ARG, "march_len < 0");
+    if (prefetch < 0 || prefetch > 3) return fail(AWCPML_ERR_ARG, "prefetch must be 0..3 (got %d)", prefetch);
     P->block_y = block_y;
     P->march_len = march_len;
+    P->prefetch = prefetch ? prefetch : 2;
     return default_tiling(P);
 }
 
-int awcpml_get_tiling(const awcpml_plan* P, int* block_y, int* march_len) {
-    if (!P || !block_y || !march_len) return fail(AWCPML_ERR_ARG, "NULL argument");
+int awcpml_get_tiling(const awcpml_plan* P, int* block_y, int* march_len, int* prefetch) {
+    if (!P || !block_y || !march_len || !prefetch) return fail(AWCPML_ERR_ARG, "NULL argument");
     *block_y = P->block_y;
     *march_len = P->march_len;
+    *prefetch = P->prefetch;
     return AWCPML_OK;
 }
 

@@ -5,11 +5,26 @@
 // (src/shared/acoustic_2D_xPU.jl:7-75, src/shared/acoustic_3D_xPU.jl:8-104):
 //
 //  * every thread owns one (x, y) column (3D) or one x position (2D) and
-//    marches along z (3D) or y (2D) through a chunk of the grid; the
-//    pressure values below, at and above the current node along the march
-//    stay in registers (pm, pc, pp), so pcur is fetched from HBM once;
-//    the x/y neighbours are plain loads that hit L1/L2 because the
-//    neighbouring lanes and waves fetch the same lines;
+//    marches along z (3D) or y (2D) through a chunk of the grid; a wave
+//    fetches each pcur line from HBM once (as the node ahead along the
+//    march) and re-reads it from L1/L2 as the node itself, the node behind
+//    and the in-plane neighbours of the neighbouring lanes and waves;
+//  * the march is software-pipelined through a ring of D register slots
+//    holding the next D nodes: every load a node needs -- pressures, pold,
+//    fact and, inside the CPML slabs, psi and xi -- is issued into the slot
+//    the current node has just consumed (so its registers are reused without
+//    copies) and stays in flight during the next D-1 nodes' arithmetic;
+//  * every loop body issues a fixed number of loads, so the compiler's
+//    memory-counter waits are exact and never drain the loads in flight:
+//    slab membership across the march is a per-wave template choice (a wave
+//    with any lane in an x slab runs the x-slab body, lanes outside the slab
+//    read a valid dummy location and do not store), the march is split into
+//    low-slab / interior / high-slab segments, the last D nodes of a segment
+//    run without refills, and the CPML coefficients of the march axis come
+//    from LDS;
+//  * addresses are 32-bit byte offsets from per-shot base pointers that are
+//    uniform across the block (scalar registers), when a shot's arrays are
+//    below 4 GiB; a 64-bit variant covers larger grids;
 //  * the psi recursions are fused into the pressure update.  psi is
 //    double-buffered (psi_rd holds step n-1, psi_wr receives step n): the
 //    thread at node m recomputes psi[m] and psi[m-1] from psi_rd and the
@@ -75,30 +90,15 @@ __device__ __forceinline__ int logical_block(int b, int grid) {
     return (b % kNumXcd) * per + b / kNumXcd;
 }
 
-// One CPML side of one axis at layer m (1 <= m <= halo) of that side's psi
-// array.  pm, pc, pp: pressure before, at and after the node along the
-// axis; d2: the node's second derivative along the axis; f: fact.
-// Returns fact*(dpsi + xi), the reference's damping term
-// (acoustic_2D_xPU.jl:45-56 for x-left; the other sides are identical up
-// to index shifts).
-template <typename T>
-__device__ __forceinline__ T cpml_term(const CpmlSide<T>& s, int64_t psi_m,
-                                       int64_t psi_m1, int64_t xi_m1, int m,
-                                       T pm, T pc, T pp, T d2, T inv_d, T f,
-                                       int64_t shot_psi, int64_t shot_xi) {
-    const T* psi_rd = s.psi_rd + shot_psi;
-    T* psi_wr = s.psi_wr + shot_psi;
-    T* xi = s.xi + shot_xi;
-    // update_psi_*!: psi[m] = b_K_h[m]*psi[m] + a_h[m]*(p[m+1]-p[m])*_d
-    const T psi_here = s.bKh[m] * psi_rd[psi_m] + s.ah[m] * (pp - pc) * inv_d;
-    const T psi_prev = s.bKh[m - 1] * psi_rd[psi_m1] + s.ah[m - 1] * (pc - pm) * inv_d;
-    psi_wr[psi_m] = psi_here;
-    if (m == 1) psi_wr[psi_m1] = psi_prev;
-    // update_p!: dpsi, in-place xi, damping
-    const T dpsi = (psi_here - psi_prev) * inv_d;
-    const T xin = s.bK[m - 1] * xi[xi_m1] + s.a[m - 1] * (d2 + dpsi);
-    xi[xi_m1] = xin;
-    return f * (dpsi + xin);
+// Loads/stores at a byte offset from a base pointer.  With a uniform base
+// and a 32-bit offset the compiler emits the scalar-base addressing form.
+template <typename T, typename Off>
+__device__ __forceinline__ T ldg(const T* base, Off byte_off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+template <typename T, typename Off>
+__device__ __forceinline__ void stg(T* base, Off byte_off, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off) = v;
 }
 
 // Classify a node along one axis: side 0/1 and layer m, or side -1.
@@ -118,145 +118,434 @@ __device__ __forceinline__ int classify(int64_t q, int64_t n, int64_t halo,
     return -1;
 }
 
-// ---------------------------------------------------------------------
-// 3D: z-march.  Block (64, BY): lane -> x, threadIdx.y -> y.
-// ---------------------------------------------------------------------
+// The six CPML coefficients one layer m needs: b_K_h[m], a_h[m],
+// b_K_h[m-1], a_h[m-1] (staggered) and b_K[m-1], a[m-1] (nodal).
 template <typename T>
+struct LayerCoef {
+    T bKh, ah, bKh1, ah1, bK1, a1;
+};
+
+template <typename T>
+__device__ __forceinline__ LayerCoef<T> layer_coef(const CpmlSide<T>& s, int m) {
+    LayerCoef<T> c;
+    c.bKh = s.bKh[m];
+    c.ah = s.ah[m];
+    c.bKh1 = s.bKh[m - 1];
+    c.ah1 = s.ah[m - 1];
+    c.bK1 = s.bK[m - 1];
+    c.a1 = s.a[m - 1];
+    return c;
+}
+
+// CPML update of one side of one axis at layer m, from the previous psi
+// values psi_old = psi[m], psi_old1 = psi[m-1] and xi_old = xi[m-1]
+// (already loaded).  pm, pc, pp: pressure before, at and after the node
+// along the axis; d2: the node's second derivative along the axis; f: fact.
+// Stores the new psi[m] (and psi[m-1] when m == 1) and xi[m-1]; returns
+// fact*(dpsi + xi), the reference's damping term
+// (acoustic_2D_xPU.jl:7-18 and 45-56 for x-left; the other sides are
+// identical up to index shifts).
+template <typename T, typename Off>
+__device__ __forceinline__ T cpml_apply(const LayerCoef<T>& c, T* psi_wr, T* xi_arr, Off psi_m,
+                                        Off psi_m1, Off xi_m1, int m, T psi_old,
+                                        T psi_old1, T xi_old, T pm, T pc, T pp, T d2,
+                                        T inv_d, T f) {
+    // update_psi_*!: psi[m] = b_K_h[m]*psi[m] + a_h[m]*(p[m+1]-p[m])*_d
+    const T psi_here = c.bKh * psi_old + c.ah * (pp - pc) * inv_d;
+    const T psi_prev = c.bKh1 * psi_old1 + c.ah1 * (pc - pm) * inv_d;
+    stg(psi_wr, psi_m, psi_here);
+    if (m == 1) stg(psi_wr, psi_m1, psi_prev);
+    // update_p!: dpsi, in-place xi, damping
+    const T dpsi = (psi_here - psi_prev) * inv_d;
+    const T xin = c.bK1 * xi_old + c.a1 * (d2 + dpsi);
+    stg(xi_arr, xi_m1, xin);
+    return f * (dpsi + xin);
+}
+
+// Pointers of one slab side for one shot, selected without memory loads.
+template <typename T>
+struct SlabPtrs {
+    const T* psi_rd;
+    T* psi_wr;
+    T* xi;
+};
+
+template <typename T>
+__device__ __forceinline__ SlabPtrs<T> slab_ptrs(const StencilArgs<T>& A, int axis, int side,
+                                                 int shot) {
+    const CpmlSide<T>& L = A.side[axis][0];
+    const CpmlSide<T>& R = A.side[axis][1];
+    SlabPtrs<T> p;
+    p.psi_rd = (side ? R.psi_rd : L.psi_rd) + shot * A.psi_stride[axis];
+    p.psi_wr = (side ? R.psi_wr : L.psi_wr) + shot * A.psi_stride[axis];
+    p.xi = (side ? R.xi : L.xi) + shot * A.xi_stride[axis];
+    return p;
+}
+
+// An in-plane CPML slab of one column (x in 3D and 2D, y in 3D): side,
+// layer, pointers, coefficients and byte offsets of psi[m], psi[m-1],
+// xi[m-1] at march position 0 plus their increments per march step.
+template <typename T, typename Off>
+struct ColSlab {
+    bool in;            // this lane's column is in the slab
+    int m;
+    SlabPtrs<T> p;
+    LayerCoef<T> c;
+    Off o_psi, o_psi1, o_xi;
+    Off st_psi, st_xi;
+};
+
+// March-axis CPML coefficients staged in LDS: for side s, ah, bKh (halo+1
+// values each) then a, bK (halo values each).
+template <typename T>
+__device__ __forceinline__ void stage_march_coef(const StencilArgs<T>& A, int axis, T* lds) {
+    const int h1 = (int)A.halo + 1;
+    const int nthreads = blockDim.x * blockDim.y;
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    for (int q = tid; q < 8 * h1; q += nthreads) {
+        const int s = q / (4 * h1);
+        const int r = q % (4 * h1);
+        const int w = r / h1, m = r % h1;
+        const CpmlSide<T>& S = A.side[axis][s];
+        T v = T(0);
+        if (w == 0) v = S.ah[m];
+        else if (w == 1) v = S.bKh[m];
+        else if (m < h1 - 1) v = (w == 2) ? S.a[m] : S.bK[m];
+        lds[q] = v;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ LayerCoef<T> lds_coef(const T* lds, int h1, int side, int m) {
+    const T* b = lds + side * 4 * h1;
+    LayerCoef<T> c;
+    c.ah = b[m];
+    c.ah1 = b[m - 1];
+    c.bKh = b[h1 + m];
+    c.bKh1 = b[h1 + m - 1];
+    c.a1 = b[2 * h1 + m - 1];
+    c.bK1 = b[3 * h1 + m - 1];
+    return c;
+}
+
+// Per-node loads of the march (one ring slot).
+template <typename T>
+struct NodeLoads {
+    T pm, pc, pp;         // pcur one node behind, at, and ahead along the march
+    T xm, xp, ym, yp;     // in-plane neighbours (ym/yp unused in 2D)
+    T po, f;              // pold, fact
+    T psa0, psa1, xia;    // first in-plane slab (x): psi[m], psi[m-1], xi[m-1]
+    T psb0, psb1, xib;    // second in-plane slab (y, 3D only)
+    T psm0, psm1, xim;    // march-axis slab
+};
+
+// Everything a column's march needs; built once per thread.
+template <typename T, typename Off, int NDIM>
+struct March {
+    const T* pcur;
+    const T* pold;
+    T* pnew;
+    const T* fact;
+    Off col;      // byte offset of the column at march position 0
+    Off sm;       // byte stride along the march axis
+    Off sy;       // byte stride along y (3D: in-plane y; 2D: == sm)
+    T i2a, i2b, i2m;  // 1/d^2 of x, y (3D), march axis
+    T ia, ib, imm;    // 1/d   of x, y (3D), march axis
+    ColSlab<T, Off> sa, sb;   // x slab, y slab (3D)
+    SlabPtrs<T> pm_side[2];   // march-axis slab pointers per side
+    int mshift[2];            // march layer = node - mshift[side]
+    const T* lds;             // march-axis coefficients
+    int h1;
+};
+
+template <typename T, typename Off, int NDIM, bool SA, bool SB, bool SM>
+__device__ __forceinline__ void load_node(const March<T, Off, NDIM>& M, int k, int mside,
+                                          NodeLoads<T>& q) {
+    constexpr Off E = (Off)sizeof(T);
+    const Off c = M.col + (Off)k * M.sm;
+    q.pm = ldg(M.pcur, c - M.sm);
+    q.pc = ldg(M.pcur, c);
+    q.pp = ldg(M.pcur, c + M.sm);
+    q.xm = ldg(M.pcur, c - E);
+    q.xp = ldg(M.pcur, c + E);
+    if (NDIM == 3) {
+        q.ym = ldg(M.pcur, c - M.sy);
+        q.yp = ldg(M.pcur, c + M.sy);
+    }
+    q.po = ldg(M.pold, c);
+    q.f = ldg(M.fact, c);
+    if (SA) {
+        q.psa0 = ldg(M.sa.p.psi_rd, M.sa.o_psi + (Off)k * M.sa.st_psi);
+        q.psa1 = ldg(M.sa.p.psi_rd, M.sa.o_psi1 + (Off)k * M.sa.st_psi);
+        q.xia = ldg((const T*)M.sa.p.xi, M.sa.o_xi + (Off)k * M.sa.st_xi);
+    }
+    if (SB) {
+        q.psb0 = ldg(M.sb.p.psi_rd, M.sb.o_psi + (Off)k * M.sb.st_psi);
+        q.psb1 = ldg(M.sb.p.psi_rd, M.sb.o_psi1 + (Off)k * M.sb.st_psi);
+        q.xib = ldg((const T*)M.sb.p.xi, M.sb.o_xi + (Off)k * M.sb.st_xi);
+    }
+    if (SM) {
+        // march-axis slab arrays share the field layout with the march axis
+        // replaced by the layer index: offset col + layer*sm
+        const int mz = k - M.mshift[mside];
+        const SlabPtrs<T>& P = M.pm_side[mside];
+        q.psm0 = ldg(P.psi_rd, M.col + (Off)mz * M.sm);
+        q.psm1 = ldg(P.psi_rd, M.col + (Off)(mz - 1) * M.sm);
+        q.xim = ldg((const T*)P.xi, M.col + (Off)(mz - 1) * M.sm);
+    }
+}
+
+template <typename T, typename Off, int NDIM, bool SA, bool SB, bool SM>
+__device__ __forceinline__ void compute_node(const March<T, Off, NDIM>& M, int k, int mside,
+                                             const NodeLoads<T>& q) {
+    const T two = T(2);
+    const T pm = q.pm, pc = q.pc, pp = q.pp;
+    const T f = q.f;
+    const T d2a = (q.xp - two * pc + q.xm) * M.i2a;
+    T d2b = T(0);
+    if (NDIM == 3) d2b = (q.yp - two * pc + q.ym) * M.i2b;
+    const T d2m = (pp - two * pc + pm) * M.i2m;
+    T damp = T(0);
+    if (SA) {
+        if (M.sa.in)
+            damp += cpml_apply(M.sa.c, M.sa.p.psi_wr, M.sa.p.xi, M.sa.o_psi + (Off)k * M.sa.st_psi,
+                               M.sa.o_psi1 + (Off)k * M.sa.st_psi, M.sa.o_xi + (Off)k * M.sa.st_xi,
+                               M.sa.m, q.psa0, q.psa1, q.xia, q.xm, pc, q.xp, d2a, M.ia, f);
+    }
+    if (NDIM == 3 && SB) {
+        damp += cpml_apply(M.sb.c, M.sb.p.psi_wr, M.sb.p.xi, M.sb.o_psi + (Off)k * M.sb.st_psi,
+                           M.sb.o_psi1 + (Off)k * M.sb.st_psi, M.sb.o_xi + (Off)k * M.sb.st_xi,
+                           M.sb.m, q.psb0, q.psb1, q.xib, q.ym, pc, q.yp, d2b, M.ib, f);
+    }
+    if (SM) {
+        const int mz = k - M.mshift[mside];
+        const LayerCoef<T> cz = lds_coef(M.lds, M.h1, mside, mz);
+        const SlabPtrs<T>& P = M.pm_side[mside];
+        damp += cpml_apply(cz, P.psi_wr, P.xi, M.col + (Off)mz * M.sm,
+                           M.col + (Off)(mz - 1) * M.sm, M.col + (Off)(mz - 1) * M.sm, mz,
+                           q.psm0, q.psm1, q.xim, pm, pc, pp, d2m, M.imm, f);
+    }
+    // update_p!: pnew = 2 pcur - pold + fact*(d2x + d2y (+ d2z)) + damp
+    T lap;
+    if (NDIM == 3) lap = d2a + d2b + d2m;   // (d2x + d2y) + d2z
+    else lap = d2a + d2m;                   // d2x + d2y
+    stg(M.pnew, M.col + (Off)k * M.sm, two * pc - q.po + f * lap + damp);
+}
+
+// March over nodes [k_from, k_to) with a fixed load set.  Each node's slot
+// holds all its operands (pcur below/at/above included), so no value is
+// carried from one slot to the next and refills never need register copies.
+template <typename T, typename Off, int NDIM, int D, bool SA, bool SB, bool SM>
+__device__ __forceinline__ void march_segment(const March<T, Off, NDIM>& M, int k_from, int k_to,
+                                              int mside) {
+    if (k_to - k_from < 2 * D) {
+        // short segment: no pipeline
+        for (int k = k_from; k < k_to; ++k) {
+            NodeLoads<T> q;
+            load_node<T, Off, NDIM, SA, SB, SM>(M, k, mside, q);
+            compute_node<T, Off, NDIM, SA, SB, SM>(M, k, mside, q);
+        }
+        return;
+    }
+    // unconditional prologue in steady-state order, so the waits the
+    // compiler derives at the loop head (a minimum over the entry and the
+    // back edge) stay close to the steady-state ones
+    NodeLoads<T> ring[D];
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+        load_node<T, Off, NDIM, SA, SB, SM>(M, k_from + u, mside, ring[u]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    int k0 = k_from;
+    // steady state: every node refills its slot with node k+D.  The
+    // scheduling barrier keeps each refill right after its node: left free,
+    // the scheduler sinks all refills to the end of the unrolled body and
+    // the next iteration waits on them at once.
+    for (; k0 + 2 * D <= k_to; k0 += D) {
+#pragma unroll
+        for (int u = 0; u < D; ++u) {
+            const int k = k0 + u;
+            compute_node<T, Off, NDIM, SA, SB, SM>(M, k, mside, ring[u]);
+            load_node<T, Off, NDIM, SA, SB, SM>(M, k + D, mside, ring[u]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    // tail: at most 2D-1 nodes, refills only while inside the segment
+#pragma unroll
+    for (int u = 0; u < 2 * D - 1; ++u) {
+        const int k = k0 + u;
+        if (k < k_to) {
+            compute_node<T, Off, NDIM, SA, SB, SM>(M, k, mside, ring[u % D]);
+            if (k + D < k_to) load_node<T, Off, NDIM, SA, SB, SM>(M, k + D, mside, ring[u % D]);
+        }
+    }
+}
+
+// Split [kbeg, kend) into low-slab / interior / high-slab segments of the
+// march axis.
+template <typename T, typename Off, int NDIM, int D, bool SA, bool SB>
+__device__ __forceinline__ void march_column(const March<T, Off, NDIM>& M, const int* active_m,
+                                             int64_t n_m, int64_t halo, int kbeg, int kend) {
+    const int lo_end = active_m[0] ? (int)min((int64_t)kend, halo + 1) : kbeg;
+    const int hi_beg = active_m[1] ? (int)max((int64_t)kbeg, n_m - halo - 1) : kend;
+    const int a = max(kbeg, lo_end), b = min(kend, max(a, hi_beg));
+    march_segment<T, Off, NDIM, D, SA, SB, true>(M, kbeg, min(lo_end, kend), 0);
+    march_segment<T, Off, NDIM, D, SA, SB, false>(M, a, b, 0);
+    march_segment<T, Off, NDIM, D, SA, SB, true>(M, max(b, hi_beg), kend, 1);
+}
+
+// Fill a ColSlab for an in-plane axis.  q: node index along the axis;
+// psi_base(m)/xi_base(m): byte offsets of layer m at march position 0;
+// steps: byte increments per march step.  Lanes outside the slab get a
+// valid dummy location (layer 1 of the low side) and in = false.
+template <typename T, typename Off>
+__device__ __forceinline__ void col_slab(const StencilArgs<T>& A, int axis, int64_t q, int64_t n,
+                                         int shot, Off psi_l0, Off psi_dl, Off xi_l0, Off xi_dl,
+                                         Off st_psi, Off st_xi, ColSlab<T, Off>& S) {
+    int m = 1;
+    int side = classify(q, n, A.halo, A.active[axis], &m);
+    S.in = side >= 0;
+    if (!S.in) {
+        side = 0;
+        m = 1;
+    }
+    S.m = m;
+    S.p = slab_ptrs(A, axis, side, shot);
+    S.c = LayerCoef<T>{};
+    if (S.in) S.c = layer_coef(A.side[axis][side], m);
+    S.o_psi = psi_l0 + (Off)m * psi_dl;
+    S.o_psi1 = psi_l0 + (Off)(m - 1) * psi_dl;
+    S.o_xi = xi_l0 + (Off)(m - 1) * xi_dl;
+    S.st_psi = st_psi;
+    S.st_xi = st_xi;
+}
+
+// ---------------------------------------------------------------------
+// 3D: z-march.  Block (64, BY): lane -> x, threadIdx.y -> y (one row per
+// wave, so y is wave-uniform).  D = ring depth.
+// ---------------------------------------------------------------------
+template <typename T, typename Off, int D>
 __global__ __launch_bounds__(256) void stencil3d(const StencilArgs<T> A) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* lds = reinterpret_cast<T*>(smem_raw);
     const int lb = logical_block(blockIdx.x, A.grid_blocks);
-    if (lb >= A.logical_blocks) return;
+    if (lb >= A.logical_blocks) return;  // whole block
+    if (A.active[2][0] || A.active[2][1]) stage_march_coef(A, 2, lds);
+    __syncthreads();
+
     int rest = lb;
     const int shot = rest % A.nshots; rest /= A.nshots;
     const int tx = rest % A.tiles_x; rest /= A.tiles_x;
     const int ty = rest % A.tiles_y;
     const int chunk = rest / A.tiles_y;
-
+    // blockDim.x == 64 == wavefront: threadIdx.y is the same on all lanes
+    const int wy = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     const int64_t i = (int64_t)tx * kTileX + threadIdx.x;
-    const int64_t j = 1 + (int64_t)ty * blockDim.y + threadIdx.y;
+    const int64_t j = 1 + (int64_t)ty * blockDim.y + wy;
     const int64_t nx = A.nx, ny = A.ny, nz = A.nz, halo = A.halo;
-    if (i < 1 || i > nx - 2 || j > ny - 2) return;
-    const int64_t kbeg = 1 + (int64_t)chunk * A.march_len;
-    const int64_t kend = min(kbeg + (int64_t)A.march_len, nz - 1);
+    if (j > ny - 2) return;
+    if (i < 1 || i > nx - 2) return;
+    const int kbeg = 1 + chunk * A.march_len;
+    const int kend = (int)min((int64_t)kbeg + A.march_len, nz - 1);
     if (kbeg >= kend) return;
 
-    const int64_t px = A.px, plane = px * ny;
+    constexpr Off E = (Off)sizeof(T);
     const int64_t fo = (int64_t)shot * A.field_stride;
-    const T* __restrict__ pcur = A.pcur + fo;
-    const T* __restrict__ pold = A.pold + fo;
-    T* __restrict__ pnew = A.pnew + fo;
-    const T* __restrict__ fact = A.fact;
-    const T two = T(2);
-    const T idx2 = A.inv_d2[0], idy2 = A.inv_d2[1], idz2 = A.inv_d2[2];
+    March<T, Off, 3> M;
+    M.pcur = A.pcur + fo;
+    M.pold = A.pold + fo;
+    M.pnew = A.pnew + fo;
+    M.fact = A.fact;
+    M.sy = (Off)A.px * E;
+    M.sm = (Off)(A.px * ny) * E;
+    M.col = ((Off)i + (Off)A.px * (Off)j) * E;
+    M.i2a = A.inv_d2[0];
+    M.i2b = A.inv_d2[1];
+    M.i2m = A.inv_d2[2];
+    M.ia = A.inv_d[0];
+    M.ib = A.inv_d[1];
+    M.imm = A.inv_d[2];
+    M.lds = lds;
+    M.h1 = (int)halo + 1;
+    const Off hp1 = (Off)halo + 1, hl = (Off)halo;
+    // psi_x: (halo+1) x ny x nz, xi_x: halo x ny x nz
+    col_slab<T, Off>(A, 0, i, nx, shot, hp1 * (Off)j * E, E, hl * (Off)j * E, E,
+                     hp1 * (Off)ny * E, hl * (Off)ny * E, M.sa);
+    // psi_y: px x (halo+1) x nz, xi_y: px x halo x nz
+    col_slab<T, Off>(A, 1, j, ny, shot, (Off)i * E, M.sy, (Off)i * E, M.sy, hp1 * M.sy,
+                     hl * M.sy, M.sb);
+    // psi_z: px x ny x (halo+1), xi_z: px x ny x halo
+    for (int s = 0; s < 2; ++s) M.pm_side[s] = slab_ptrs(A, 2, s, shot);
+    M.mshift[0] = 0;
+    M.mshift[1] = (int)(nz - halo - 2);
 
-    int mx = 0, my = 0;
-    const int sx = classify(i, nx, halo, A.active[0], &mx);
-    const int sy = classify(j, ny, halo, A.active[1], &my);
-    const int64_t hp1 = halo + 1;
-
-    const int64_t col = i + px * j;
-    T pm = pcur[col + plane * (kbeg - 1)];
-    T pc = pcur[col + plane * kbeg];
-    for (int64_t k = kbeg; k < kend; ++k) {
-        const int64_t idx = col + plane * k;
-        const T pp = pcur[idx + plane];
-        const T xm = pcur[idx - 1], xp = pcur[idx + 1];
-        const T ym = pcur[idx - px], yp = pcur[idx + px];
-        const T po = pold[idx];
-        const T f = fact[idx];
-        const T d2x = (xp - two * pc + xm) * idx2;
-        const T d2y = (yp - two * pc + ym) * idy2;
-        const T d2z = (pp - two * pc + pm) * idz2;
-        T damp = T(0);
-        if (sx >= 0) {
-            // psi_x: (halo+1) x ny x nz ; xi_x: halo x ny x nz
-            const int64_t r = j + ny * k;
-            damp += cpml_term(A.side[0][sx], mx + hp1 * r, (mx - 1) + hp1 * r,
-                              (mx - 1) + halo * r, mx, xm, pc, xp, d2x, A.inv_d[0], f,
-                              shot * A.psi_stride[0], shot * A.xi_stride[0]);
-        }
-        if (sy >= 0) {
-            // psi_y: nx x (halo+1) x nz ; xi_y: nx x halo x nz (pitch px)
-            damp += cpml_term(A.side[1][sy], i + px * (my + hp1 * k),
-                              i + px * ((my - 1) + hp1 * k), i + px * ((my - 1) + halo * k),
-                              my, ym, pc, yp, d2y, A.inv_d[1], f,
-                              shot * A.psi_stride[1], shot * A.xi_stride[1]);
-        }
-        int mz = 0;
-        const int sz = classify(k, nz, halo, A.active[2], &mz);
-        if (sz >= 0) {
-            // psi_z: nx x ny x (halo+1) ; xi_z: nx x ny x halo (pitch px)
-            damp += cpml_term(A.side[2][sz], col + plane * mz, col + plane * (mz - 1),
-                              col + plane * (mz - 1), mz, pm, pc, pp, d2z, A.inv_d[2], f,
-                              shot * A.psi_stride[2], shot * A.xi_stride[2]);
-        }
-        pnew[idx] = two * pc - po + f * (d2x + d2y + d2z) + damp;
-        pm = pc;
-        pc = pp;
+    const bool xs = __any(M.sa.in);
+    const bool ys = M.sb.in;  // wave-uniform
+    if (xs) {
+        if (ys) march_column<T, Off, 3, D, true, true>(M, A.active[2], nz, halo, kbeg, kend);
+        else march_column<T, Off, 3, D, true, false>(M, A.active[2], nz, halo, kbeg, kend);
+    } else {
+        if (ys) march_column<T, Off, 3, D, false, true>(M, A.active[2], nz, halo, kbeg, kend);
+        else march_column<T, Off, 3, D, false, false>(M, A.active[2], nz, halo, kbeg, kend);
     }
 }
 
 // ---------------------------------------------------------------------
 // 2D: y-march.  Block (64, BY): lane -> x, threadIdx.y -> march chunk.
 // ---------------------------------------------------------------------
-template <typename T>
+template <typename T, typename Off, int D>
 __global__ __launch_bounds__(256) void stencil2d(const StencilArgs<T> A) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* lds = reinterpret_cast<T*>(smem_raw);
     const int lb = logical_block(blockIdx.x, A.grid_blocks);
-    if (lb >= A.logical_blocks) return;
+    if (lb >= A.logical_blocks) return;  // whole block
+    if (A.active[1][0] || A.active[1][1]) stage_march_coef(A, 1, lds);
+    __syncthreads();
+
     int rest = lb;
     const int shot = rest % A.nshots; rest /= A.nshots;
     const int tx = rest % A.tiles_x; rest /= A.tiles_x;
     const int ty = rest;  // group of blockDim.y chunks
-    const int chunk = ty * blockDim.y + threadIdx.y;
-
+    const int wy = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    const int chunk = ty * blockDim.y + wy;
     const int64_t i = (int64_t)tx * kTileX + threadIdx.x;
     const int64_t nx = A.nx, ny = A.ny, halo = A.halo;
-    if (i < 1 || i > nx - 2 || chunk >= A.nchunks) return;
-    const int64_t jbeg = 1 + (int64_t)chunk * A.march_len;
-    const int64_t jend = min(jbeg + (int64_t)A.march_len, ny - 1);
+    if (chunk >= A.nchunks) return;
+    if (i < 1 || i > nx - 2) return;
+    const int jbeg = 1 + chunk * A.march_len;
+    const int jend = (int)min((int64_t)jbeg + A.march_len, ny - 1);
     if (jbeg >= jend) return;
 
-    const int64_t px = A.px;
+    constexpr Off E = (Off)sizeof(T);
     const int64_t fo = (int64_t)shot * A.field_stride;
-    const T* __restrict__ pcur = A.pcur + fo;
-    const T* __restrict__ pold = A.pold + fo;
-    T* __restrict__ pnew = A.pnew + fo;
-    const T* __restrict__ fact = A.fact;
-    const T two = T(2);
-    const T idx2 = A.inv_d2[0], idy2 = A.inv_d2[1];
+    March<T, Off, 2> M;
+    M.pcur = A.pcur + fo;
+    M.pold = A.pold + fo;
+    M.pnew = A.pnew + fo;
+    M.fact = A.fact;
+    M.sy = (Off)A.px * E;
+    M.sm = M.sy;
+    M.col = (Off)i * E;
+    M.i2a = A.inv_d2[0];
+    M.i2b = T(0);
+    M.i2m = A.inv_d2[1];
+    M.ia = A.inv_d[0];
+    M.ib = T(0);
+    M.imm = A.inv_d[1];
+    M.lds = lds;
+    M.h1 = (int)halo + 1;
+    const Off hp1 = (Off)halo + 1, hl = (Off)halo;
+    // psi_x: (halo+1) x ny, xi_x: halo x ny
+    col_slab<T, Off>(A, 0, i, nx, shot, 0, E, 0, E, hp1 * E, hl * E, M.sa);
+    M.sb = M.sa;
+    M.sb.in = false;
+    // psi_y: px x (halo+1), xi_y: px x halo
+    for (int s = 0; s < 2; ++s) M.pm_side[s] = slab_ptrs(A, 1, s, shot);
+    M.mshift[0] = 0;
+    M.mshift[1] = (int)(ny - halo - 2);
 
-    int mx = 0;
-    const int sx = classify(i, nx, halo, A.active[0], &mx);
-    const int64_t hp1 = halo + 1;
-
-    T pm = pcur[i + px * (jbeg - 1)];
-    T pc = pcur[i + px * jbeg];
-    for (int64_t j = jbeg; j < jend; ++j) {
-        const int64_t idx = i + px * j;
-        const T pp = pcur[idx + px];
-        const T xm = pcur[idx - 1], xp = pcur[idx + 1];
-        const T po = pold[idx];
-        const T f = fact[idx];
-        const T d2x = (xp - two * pc + xm) * idx2;
-        const T d2y = (pp - two * pc + pm) * idy2;
-        T damp = T(0);
-        if (sx >= 0) {
-            // psi_x: (halo+1) x ny ; xi_x: halo x ny
-            damp += cpml_term(A.side[0][sx], mx + hp1 * j, (mx - 1) + hp1 * j,
-                              (mx - 1) + halo * j, mx, xm, pc, xp, d2x, A.inv_d[0], f,
-                              shot * A.psi_stride[0], shot * A.xi_stride[0]);
-        }
-        int my = 0;
-        const int sy = classify(j, ny, halo, A.active[1], &my);
-        if (sy >= 0) {
-            // psi_y: nx x (halo+1) ; xi_y: nx x halo (pitch px)
-            damp += cpml_term(A.side[1][sy], i + px * my, i + px * (my - 1),
-                              i + px * (my - 1), my, pm, pc, pp, d2y, A.inv_d[1], f,
-                              shot * A.psi_stride[1], shot * A.xi_stride[1]);
-        }
-        pnew[idx] = two * pc - po + f * (d2x + d2y) + damp;
-        pm = pc;
-        pc = pp;
-    }
+    if (__any(M.sa.in)) march_column<T, Off, 2, D, true, false>(M, A.active[1], ny, halo, jbeg, jend);
+    else march_column<T, Off, 2, D, false, false>(M, A.active[1], ny, halo, jbeg, jend);
 }
 
 // ---------------------------------------------------------------------

@@ -138,10 +138,13 @@ int awcpml_time_stencil(awcpml_plan *plan, int nlaunches, double *ms_total);
 int awcpml_time_advance(awcpml_plan *plan, int64_t nsteps, double *ms_total);
 
 /* Launch geometry of the stencil kernel: threads per block along y
- * (1, 2 or 4) and the length of the z (3D) or y (2D) march per block.
- * 0 keeps the library's choice.  Used by the autotuner. */
-int awcpml_set_tiling(awcpml_plan *plan, int block_y, int march_len);
-int awcpml_get_tiling(const awcpml_plan *plan, int *block_y, int *march_len);
+ * (1, 2 or 4), the length of the z (3D) or y (2D) march per block, and the
+ * software-pipeline depth of the march (1..3: loads issued up to prefetch-1
+ * nodes ahead).  0 keeps the library's choice.  Used by the autotuner; no
+ * reference counterpart (ParallelStencil picks its launch shape itself). */
+int awcpml_set_tiling(awcpml_plan *plan, int block_y, int march_len, int prefetch);
+int awcpml_get_tiling(const awcpml_plan *plan, int *block_y, int *march_len,
+                      int *prefetch);
 
 /* Message of the calling thread's last failed call ("" if none). */
 const char *awcpml_last_error(void);
