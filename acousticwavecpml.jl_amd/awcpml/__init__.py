@@ -3,11 +3,14 @@
 Public surface (mirrors the reference's exports and solver entry points):
   Sources, Receivers, rickersource1D, calc_Kab_CPML   (src/utils.jl)
   solve2D, solve3D      (src/shared/acoustic_{2D,3D}_xPU.jl)
-  Plan                  the C-ABI plan (multi-shot, FP64/FP32)
+  Plan                  the C-ABI plan (multi-shot, FP64/FP32, graph capture)
+  Batch                 several plans (uneven grids) advanced by one launch per step
+  autotune              launch-shape autotuner with a JSON cache
 """
 from .host import Receivers, Sources, calc_Kab_CPML, rickersource1D, setup  # noqa: F401
-from .solver import Plan, solve2D, solve3D  # noqa: F401
+from .solver import Batch, Plan, solve2D, solve3D  # noqa: F401
+from .autotune import autotune  # noqa: F401
 from . import _lib  # noqa: F401
 
 __all__ = ["Sources", "Receivers", "rickersource1D", "calc_Kab_CPML", "setup", "Plan",
-           "solve2D", "solve3D"]
+           "Batch", "autotune", "solve2D", "solve3D"]

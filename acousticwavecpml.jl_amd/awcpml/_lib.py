@@ -48,6 +48,14 @@ SIGNATURES = {
     "awcpml_get_tiling": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_int)]),
+    "awcpml_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "awcpml_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(_vp)]),
+    "awcpml_batch_destroy": (ctypes.c_int, [_vp]),
+    "awcpml_batch_reset": (ctypes.c_int, [_vp]),
+    "awcpml_batch_advance": (ctypes.c_int, [_vp, ctypes.c_int64]),
+    "awcpml_batch_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "awcpml_batch_time_advance": (ctypes.c_int, [_vp, ctypes.c_int64, _dp]),
+    "awcpml_batch_time_stencil": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "awcpml_last_error": (ctypes.c_char_p, []),
     "awcpml_version": (ctypes.c_char_p, []),
 }

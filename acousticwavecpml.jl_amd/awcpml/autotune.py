@@ -1,0 +1,73 @@
+"""Launch-shape autotuner for the stencil kernel.
+
+For a plan, time each candidate (block_y, march_len, prefetch) with HIP
+events around a few stencil launches (``Plan.time_stencil``), keep the
+fastest, apply it to the plan and remember it in a JSON cache keyed by the
+problem's shape, halo, precision and shot count.  The wavefield state is
+reset afterwards.
+
+    from awcpml.autotune import autotune
+    best = autotune(plan)            # {"block_y": 4, "march_len": 148, ...}
+"""
+import json
+import os
+
+DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "awcpml", "tuning.json")
+
+
+def candidates(plan):
+    """Launch shapes worth timing for this plan's grid."""
+    n_march = plan.n[-1] - 2 if plan.ndim == 3 else plan.n[1] - 2
+    lens = sorted({min(n_march, v) for v in (8, 16, 32, 64, (n_march + 1) // 2, n_march)})
+    out = []
+    for by in (1, 2, 4):
+        for ml in lens:
+            for pf in (1, 2, 3):
+                if pf > 1 and ml < 2 * pf:
+                    continue
+                out.append((by, ml, pf))
+    return out
+
+
+def _key(plan):
+    return "%dD:%s:h%d:%s:s%d" % (plan.ndim, "x".join(map(str, plan.n)), plan.halo,
+                                   "f64" if plan.dtype == 0 else "f32", plan.nshots)
+
+
+def _load(path):
+    try:
+        with open(path) as fh:
+            return json.load(fh)
+    except (OSError, ValueError):
+        return {}
+
+
+def autotune(plan, cands=None, launches=8, cache=DEFAULT_CACHE, use_cache=True):
+    """Pick the fastest launch shape for ``plan`` and apply it."""
+    key = _key(plan)
+    db = _load(cache) if cache else {}
+    if use_cache and key in db:
+        best = db[key]
+        plan.set_tiling(best["block_y"], best["march_len"], best["prefetch"])
+        return dict(best, cached=True)
+    results = []
+    for by, ml, pf in (cands or candidates(plan)):
+        plan.set_tiling(by, ml, pf)
+        plan.reset()
+        plan.time_stencil(2)  # warm-up
+        ms = plan.time_stencil(launches) / launches
+        results.append((ms, by, ml, pf))
+    plan.reset()
+    ms, by, ml, pf = min(results)
+    plan.set_tiling(by, ml, pf)
+    best = {"block_y": by, "march_len": ml, "prefetch": pf, "ms_per_launch": ms,
+            "tried": len(results)}
+    if cache:
+        db[key] = best
+        try:
+            os.makedirs(os.path.dirname(cache), exist_ok=True)
+            with open(cache, "w") as fh:
+                json.dump(db, fh, indent=1)
+        except OSError:
+            pass
+    return dict(best, cached=False)

@@ -98,6 +98,10 @@ class Plan:
             self._nrecs = {}
         self._nrecs[shot] = pos.shape[0]
 
+    def set_graph(self, enable=True):
+        """Capture the time loop in HIP graphs (replayed on later solves)."""
+        _lib.check(_lib.load().awcpml_set_graph(self._h, int(bool(enable))))
+
     def set_tiling(self, block_y=0, march_len=0, prefetch=0):
         _lib.check(_lib.load().awcpml_set_tiling(self._h, block_y, march_len, prefetch))
 
@@ -155,6 +159,58 @@ class Plan:
         P.set_sources(0, S.possrcs, S.dt2srctf)
         P.set_receivers(0, S.posrecs)
         return P
+
+
+class Batch:
+    """Several plans (different grids/models, same ndim and dtype) advanced
+    by one stencil launch and one source/receiver launch per time step.
+    The plans stay usable for reading results (``traces``/``field``)."""
+
+    def __init__(self, plans):
+        L = _lib.load()
+        self.plans = list(plans)
+        arr = (ctypes.c_void_p * len(self.plans))(*[p._h for p in self.plans])
+        h = ctypes.c_void_p()
+        _lib.check(L.awcpml_batch_create(arr, len(self.plans), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().awcpml_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def reset(self):
+        _lib.check(_lib.load().awcpml_batch_reset(self._h))
+
+    def advance(self, nsteps):
+        _lib.check(_lib.load().awcpml_batch_advance(self._h, int(nsteps)))
+
+    def run(self):
+        """Reset and run every plan to its own nt."""
+        self.reset()
+        self.advance(max(p.nt for p in self.plans))
+
+    def set_graph(self, enable=True):
+        _lib.check(_lib.load().awcpml_batch_set_graph(self._h, int(bool(enable))))
+
+    def time_advance(self, nsteps):
+        ms = ctypes.c_double()
+        _lib.check(_lib.load().awcpml_batch_time_advance(self._h, int(nsteps), ctypes.byref(ms)))
+        return ms.value
+
+    def time_stencil(self, nlaunches):
+        ms = ctypes.c_double()
+        _lib.check(_lib.load().awcpml_batch_time_stencil(self._h, int(nlaunches), ctypes.byref(ms)))
+        return ms.value
 
 
 def _effective_bytes(n, halo, esize):

@@ -52,6 +52,16 @@ int fail(int code, const char* fmt, ...) {
 
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
+struct GraphEntry {
+    int64_t it0, n;
+    hipGraphExec_t exec;
+};
+
+void drop_graphs(std::vector<GraphEntry>& g) {
+    for (auto& e : g) (void)hipGraphExecDestroy(e.exec);
+    g.clear();
+}
+
 }  // namespace
 
 struct awcpml_plan {
@@ -93,6 +103,9 @@ struct awcpml_plan {
     int par = 0;      // psi[..][par] = previous step
 
     hipStream_t stream = nullptr;
+    // captured step sequences (hipGraph), keyed by first step and length
+    bool use_graph = false;
+    std::vector<GraphEntry> graphs;
     int block_y = 0, march_len = 0;
     int prefetch = 2;  // software-pipeline depth of the march (1..3)
     int device = 0;
@@ -157,8 +170,10 @@ int default_tiling(awcpml_plan* P) {
     return AWCPML_OK;
 }
 
+// Kernel arguments for rotation state (rot, par): p[rot] = pold,
+// p[rot+1] = pcur, p[rot+2] = pnew; psi[par] = previous step.
 template <typename T>
-StencilArgs<T> make_args(const awcpml_plan* P) {
+StencilArgs<T> make_args(const awcpml_plan* P, int rot, int par) {
     StencilArgs<T> A;
     memset(&A, 0, sizeof A);
     A.nx = P->n[0];
@@ -167,9 +182,9 @@ StencilArgs<T> make_args(const awcpml_plan* P) {
     A.px = P->px;
     A.halo = P->halo;
     A.field_stride = P->field_elems;
-    A.pold = (const T*)P->p[P->rot % 3];
-    A.pcur = (const T*)P->p[(P->rot + 1) % 3];
-    A.pnew = (T*)P->p[(P->rot + 2) % 3];
+    A.pold = (const T*)P->p[rot % 3];
+    A.pcur = (const T*)P->p[(rot + 1) % 3];
+    A.pnew = (T*)P->p[(rot + 2) % 3];
     A.fact = (const T*)P->fact;
     for (int a = 0; a < 3; ++a) {
         A.inv_d[a] = (T)P->inv_d[a];
@@ -179,8 +194,8 @@ StencilArgs<T> make_args(const awcpml_plan* P) {
         for (int s = 0; s < 2; ++s) {
             A.active[a][s] = (a < P->ndim) ? P->active[a][s] : 0;
             awcpml::CpmlSide<T>& S = A.side[a][s];
-            S.psi_rd = (const T*)P->psi[a][s][P->par];
-            S.psi_wr = (T*)P->psi[a][s][P->par ^ 1];
+            S.psi_rd = (const T*)P->psi[a][s][par];
+            S.psi_wr = (T*)P->psi[a][s][par ^ 1];
             S.xi = (T*)P->xi[a][s];
             S.a = (const T*)P->coef[a][0 + s];
             S.bK = (const T*)P->coef[a][2 + s];
@@ -228,7 +243,7 @@ void launch_depth(const awcpml_plan* P, const StencilArgs<T>& A) {
 
 template <typename T>
 int launch_stencil_t(awcpml_plan* P) {
-    StencilArgs<T> A = make_args<T>(P);
+    StencilArgs<T> A = make_args<T>(P, P->rot, P->par);
     // 32-bit byte offsets when one shot's field fits in 4 GiB (the slabs
     // are smaller than a field)
     if ((uint64_t)P->field_elems * sizeof(T) < (1ull << 32))
@@ -244,12 +259,24 @@ int launch_stencil(awcpml_plan* P) {
 }
 
 template <typename T>
+awcpml::IoArgs<T> make_io(const awcpml_plan* P, int rot) {
+    awcpml::IoArgs<T> A;
+    A.pnew = (T*)P->p[(rot + 2) % 3];
+    A.field_stride = P->field_elems;
+    A.src_lin = (const int64_t*)P->d_src_lin;
+    A.src_wav = (const T*)P->d_src_wav;
+    A.src_start = (const int*)P->d_src_start;
+    A.src_seq = (const int*)P->d_src_seq;
+    A.rec_lin = (const int64_t*)P->d_rec_lin;
+    A.traces = (T*)P->d_traces;
+    A.rec_start = (const int*)P->d_rec_start;
+    A.nt = P->nt;
+    return A;
+}
+
+template <typename T>
 int launch_io_t(awcpml_plan* P, int64_t it0) {
-    T* pnew = (T*)P->p[(P->rot + 2) % 3];
-    awcpml::sources_receivers<T><<<dim3(P->nshots), dim3(256), 0, P->stream>>>(
-        pnew, P->field_elems, (const int64_t*)P->d_src_lin, (const T*)P->d_src_wav,
-        (const int*)P->d_src_start, (const int*)P->d_src_seq, (const int64_t*)P->d_rec_lin,
-        (T*)P->d_traces, (const int*)P->d_rec_start, P->nt, it0);
+    awcpml::sources_receivers<T><<<dim3(P->nshots), dim3(256), 0, P->stream>>>(make_io<T>(P, P->rot), it0);
     HIP_TRY(hipGetLastError());
     return AWCPML_OK;
 }
@@ -279,6 +306,7 @@ int sync_io(awcpml_plan* P) {
         rlin.insert(rlin.end(), P->rec_lin[s].begin(), P->rec_lin[s].end());
         rstart.push_back((int)rlin.size());
     }
+    drop_graphs(P->graphs);  // they reference the buffers below
     dfree(P->d_src_lin);
     dfree(P->d_src_wav);
     dfree(P->d_src_start);
@@ -333,6 +361,70 @@ int do_step(awcpml_plan* P, int64_t it0) {
     return AWCPML_OK;
 }
 
+// Capture `n` steps starting at step it0 on `stream` into an executable
+// graph.  step(q) enqueues step it0+q.  No memset is recorded.
+template <typename StepFn>
+int capture_steps(hipStream_t stream, int64_t n, StepFn step, hipGraphExec_t* out) {
+    HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    int st = AWCPML_OK;
+    for (int64_t q = 0; q < n && !st; ++q) st = step(q);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(stream, &graph);
+    if (st) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    if (e != hipSuccess) return fail(AWCPML_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(AWCPML_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    return AWCPML_OK;
+}
+
+GraphEntry* find_graph(std::vector<GraphEntry>& g, int64_t it0, int64_t n) {
+    for (auto& e : g)
+        if (e.it0 == it0 && e.n == n) return &e;
+    if (g.size() >= 4) {
+        (void)hipGraphExecDestroy(g.front().exec);
+        g.erase(g.begin());
+    }
+    return nullptr;
+}
+
+// Enqueue n steps from the plan's current state (no synchronisation).  With
+// graphs enabled the sequence is captured once per (first step, length)
+// and replayed afterwards; the rotation state then follows from the step
+// counter (it never does after awcpml_forward, which falls back to plain
+// launches).
+int run_steps(awcpml_plan* P, int64_t n) {
+    if (n <= 0) return AWCPML_OK;
+    if (P->use_graph && P->rot == (int)(P->it % 3) && P->par == (int)(P->it % 2)) {
+        GraphEntry* g = find_graph(P->graphs, P->it, n);
+        if (!g) {
+            const int rot0 = P->rot, par0 = P->par;
+            const int64_t it0 = P->it;
+            hipGraphExec_t exec;
+            int st = capture_steps(P->stream, n, [&](int64_t q) { return do_step(P, it0 + q); }, &exec);
+            P->rot = rot0;
+            P->par = par0;
+            if (st) return st;
+            P->graphs.push_back({it0, n, exec});
+            g = &P->graphs.back();
+        }
+        HIP_TRY(hipGraphLaunch(g->exec, P->stream));
+        P->it += n;
+        P->rot = (int)((P->rot + n) % 3);
+        P->par = (int)((P->par + n) % 2);
+        return AWCPML_OK;
+    }
+    for (int64_t q = 0; q < n; ++q) {
+        int st = do_step(P, P->it);
+        if (st) return st;
+        ++P->it;
+    }
+    return AWCPML_OK;
+}
+
 int64_t lin_index(const awcpml_plan* P, const int64_t* pos, int count, int item) {
     int64_t i = pos[item] - 1;
     int64_t j = pos[(int64_t)count + item] - 1;
@@ -348,6 +440,141 @@ int check_positions(const awcpml_plan* P, const int64_t* pos, int count, const c
                 return fail(AWCPML_ERR_RANGE, "At least one %s is not inside the model! (%s %d axis %d index %lld)",
                             what, what, r, a, (long long)v);
         }
+    return AWCPML_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------
+// Batches: several plans (different grids, same ndim and precision)
+// advanced by one stencil launch and one source/receiver launch per step.
+// ---------------------------------------------------------------------
+struct awcpml_batch {
+    std::vector<awcpml_plan*> plans;
+    int ndim = 0, dtype = 0, block_y = 4, prefetch = 2, halo_max = 0;
+    bool use32 = true, has_io = false;
+    hipStream_t stream = nullptr;
+    void* d_args = nullptr;  // 6 rotation variants x nplans stencil arguments
+    void* d_io = nullptr;    // 3 rotation variants x nplans io arguments
+    std::vector<int> block_start, shot_start;
+    int grid_blocks = 0;
+    int64_t it = 0, nt_max = 0;
+    bool use_graph = false;
+    std::vector<GraphEntry> graphs;
+};
+
+namespace {
+
+template <typename T>
+int batch_build_t(awcpml_batch* B) {
+    const int np = (int)B->plans.size();
+    std::vector<StencilArgs<T>> args((size_t)6 * np);
+    std::vector<awcpml::IoArgs<T>> io((size_t)3 * np);
+    for (int v = 0; v < 6; ++v)
+        for (int q = 0; q < np; ++q) args[(size_t)v * np + q] = make_args<T>(B->plans[q], v % 3, v % 2);
+    for (int r = 0; r < 3; ++r)
+        for (int q = 0; q < np; ++q) io[(size_t)r * np + q] = make_io<T>(B->plans[q], r);
+    B->block_start.assign(1, 0);
+    B->shot_start.assign(1, 0);
+    for (int q = 0; q < np; ++q) {
+        B->block_start.push_back(B->block_start.back() + args[q].logical_blocks);
+        B->shot_start.push_back(B->shot_start.back() + B->plans[q]->nshots);
+    }
+    B->grid_blocks = (int)round_up(B->block_start.back(), awcpml::kNumXcd);
+    int st;
+    if ((st = dalloc(&B->d_args, args.size() * sizeof(StencilArgs<T>)))) return st;
+    if ((st = dalloc(&B->d_io, io.size() * sizeof(awcpml::IoArgs<T>)))) return st;
+    HIP_TRY(hipMemcpyAsync(B->d_args, args.data(), args.size() * sizeof(StencilArgs<T>),
+                           hipMemcpyHostToDevice, B->stream));
+    HIP_TRY(hipMemcpyAsync(B->d_io, io.data(), io.size() * sizeof(awcpml::IoArgs<T>),
+                           hipMemcpyHostToDevice, B->stream));
+    HIP_TRY(hipStreamSynchronize(B->stream));
+    return AWCPML_OK;
+}
+
+template <typename T, typename Off, int D>
+void batch_launch_variant(const awcpml_batch* B, const awcpml::BatchArgs<T>& A) {
+    dim3 block(awcpml::kTileX, B->block_y);
+    const size_t lds = (size_t)8 * (B->halo_max + 1) * sizeof(T);
+    if (B->ndim == 3)
+        awcpml::stencil_batch<T, Off, D, 3><<<dim3(B->grid_blocks), block, lds, B->stream>>>(A);
+    else
+        awcpml::stencil_batch<T, Off, D, 2><<<dim3(B->grid_blocks), block, lds, B->stream>>>(A);
+}
+
+template <typename T>
+int batch_step_t(awcpml_batch* B, int64_t it0, bool with_io) {
+    const int np = (int)B->plans.size();
+    awcpml::BatchArgs<T> A;
+    memset(&A, 0, sizeof A);
+    A.table = (const StencilArgs<T>*)B->d_args + (size_t)(it0 % 6) * np;
+    A.it0 = it0;
+    A.nplans = np;
+    A.grid_blocks = B->grid_blocks;
+    for (int q = 0; q < np; ++q) A.nt[q] = B->plans[q]->nt;
+    for (int q = 0; q <= np; ++q) A.block_start[q] = B->block_start[q];
+    if (B->use32) {
+        switch (B->prefetch) {
+            case 1: batch_launch_variant<T, uint32_t, 1>(B, A); break;
+            case 3: batch_launch_variant<T, uint32_t, 3>(B, A); break;
+            default: batch_launch_variant<T, uint32_t, 2>(B, A); break;
+        }
+    } else {
+        switch (B->prefetch) {
+            case 1: batch_launch_variant<T, uint64_t, 1>(B, A); break;
+            case 3: batch_launch_variant<T, uint64_t, 3>(B, A); break;
+            default: batch_launch_variant<T, uint64_t, 2>(B, A); break;
+        }
+    }
+    HIP_TRY(hipGetLastError());
+    if (with_io && B->has_io) {
+        awcpml::IoBatchArgs<T> I;
+        memset(&I, 0, sizeof I);
+        I.table = (const awcpml::IoArgs<T>*)B->d_io + (size_t)(it0 % 3) * np;
+        I.it0 = it0;
+        I.nplans = np;
+        for (int q = 0; q <= np; ++q) I.shot_start[q] = B->shot_start[q];
+        awcpml::sources_receivers_batch<T><<<dim3(B->shot_start[np]), dim3(256), 0, B->stream>>>(I);
+        HIP_TRY(hipGetLastError());
+    }
+    return AWCPML_OK;
+}
+
+int batch_step(awcpml_batch* B, int64_t it0, bool with_io = true) {
+    return B->dtype == AWCPML_F64 ? batch_step_t<double>(B, it0, with_io)
+                                  : batch_step_t<float>(B, it0, with_io);
+}
+
+// After the batch ran to step B->it, every plan is at min(it, its nt).
+void batch_sync_plans(awcpml_batch* B) {
+    for (auto* P : B->plans) {
+        P->it = std::min(B->it, P->nt);
+        P->rot = (int)(P->it % 3);
+        P->par = (int)(P->it % 2);
+    }
+}
+
+int batch_run_steps(awcpml_batch* B, int64_t n) {
+    if (n <= 0) return AWCPML_OK;
+    if (B->use_graph) {
+        GraphEntry* g = find_graph(B->graphs, B->it, n);
+        if (!g) {
+            const int64_t it0 = B->it;
+            hipGraphExec_t exec;
+            int st = capture_steps(B->stream, n, [&](int64_t q) { return batch_step(B, it0 + q); }, &exec);
+            if (st) return st;
+            B->graphs.push_back({it0, n, exec});
+            g = &B->graphs.back();
+        }
+        HIP_TRY(hipGraphLaunch(g->exec, B->stream));
+        B->it += n;
+        return AWCPML_OK;
+    }
+    for (int64_t q = 0; q < n; ++q) {
+        int st = batch_step(B, B->it);
+        if (st) return st;
+        ++B->it;
+    }
     return AWCPML_OK;
 }
 
@@ -435,6 +662,7 @@ int awcpml_plan_create(int ndim, int64_t nx, int64_t ny, int64_t nz, int halo, i
 int awcpml_plan_destroy(awcpml_plan* P) {
     if (!P) return AWCPML_OK;
     if (P->stream) (void)hipStreamSynchronize(P->stream);
+    drop_graphs(P->graphs);
     for (int b = 0; b < 3; ++b) dfree(P->p[b]);
     dfree(P->fact);
     for (int a = 0; a < 3; ++a) {
@@ -503,6 +731,7 @@ int awcpml_set_cpml(awcpml_plan* P, int axis, const double* a_l, const double* a
         P->active[axis][s] = (h > 0 && !idle) ? 1 : 0;
     }
     P->cpml_set[axis] = true;
+    drop_graphs(P->graphs);
     return AWCPML_OK;
 }
 
@@ -577,10 +806,7 @@ int awcpml_advance(awcpml_plan* P, int64_t nsteps) {
     if (P->it + nsteps > P->nt)
         return fail(AWCPML_ERR_STATE, "advance past nt: %lld + %lld > %lld", (long long)P->it,
                     (long long)nsteps, (long long)P->nt);
-    for (int64_t s = 0; s < nsteps; ++s) {
-        if ((st = do_step(P, P->it))) return st;
-        ++P->it;
-    }
+    if ((st = run_steps(P, nsteps))) return st;
     HIP_TRY(hipStreamSynchronize(P->stream));
     return AWCPML_OK;
 }
@@ -696,11 +922,19 @@ int awcpml_time_advance(awcpml_plan* P, int64_t nsteps, double* ms_total) {
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    HIP_TRY(hipEventRecord(e0, P->stream));
-    for (int64_t s = 0; s < nsteps; ++s) {
-        if ((st = do_step(P, P->it))) break;
-        ++P->it;
+    if (P->use_graph && nsteps > 0 && !find_graph(P->graphs, P->it, nsteps)) {
+        // capture outside the timed region
+        const int64_t it0 = P->it;
+        const int rot0 = P->rot, par0 = P->par;
+        hipGraphExec_t exec;
+        st = capture_steps(P->stream, nsteps, [&](int64_t q) { return do_step(P, it0 + q); }, &exec);
+        P->rot = rot0;
+        P->par = par0;
+        if (st) return st;
+        P->graphs.push_back({it0, nsteps, exec});
     }
+    HIP_TRY(hipEventRecord(e0, P->stream));
+    st = run_steps(P, nsteps);
     HIP_TRY(hipEventRecord(e1, P->stream));
     HIP_TRY(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -709,6 +943,15 @@ int awcpml_time_advance(awcpml_plan* P, int64_t nsteps, double* ms_total) {
     (void)hipEventDestroy(e1);
     *ms_total = ms;
     return st;
+}
+
+int awcpml_set_graph(awcpml_plan* P, int enable) {
+    g_err.clear();
+    int st = check_plan(P);
+    if (st) return st;
+    P->use_graph = enable != 0;
+    if (!P->use_graph) drop_graphs(P->graphs);
+    return AWCPML_OK;
 }
 
 int awcpml_set_tiling(awcpml_plan* P, int block_y, int march_len, int prefetch) {
@@ -722,7 +965,151 @@ int awcpml_set_tiling(awcpml_plan* P, int block_y, int march_len, int prefetch) 
     P->block_y = block_y;
     P->march_len = march_len;
     P->prefetch = prefetch ? prefetch : 2;
+    drop_graphs(P->graphs);
     return default_tiling(P);
+}
+
+int awcpml_batch_create(awcpml_plan* const* plans, int count, awcpml_batch** out) {
+    g_err.clear();
+    if (!out || !plans) return fail(AWCPML_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    if (count < 1 || count > awcpml::kMaxBatch)
+        return fail(AWCPML_ERR_ARG, "a batch holds 1..%d plans (got %d)", awcpml::kMaxBatch, count);
+    for (int q = 0; q < count; ++q) {
+        if (!plans[q]) return fail(AWCPML_ERR_ARG, "plan %d is NULL", q);
+        for (int r = 0; r < q; ++r)
+            if (plans[r] == plans[q]) return fail(AWCPML_ERR_ARG, "plan %d appears twice", q);
+        if (plans[q]->ndim != plans[0]->ndim || plans[q]->dtype != plans[0]->dtype)
+            return fail(AWCPML_ERR_ARG, "plans of a batch must share ndim and dtype (plan %d differs)", q);
+        int st = ready(plans[q]);
+        if (st) return st;
+        if (plans[q]->block_y != plans[0]->block_y || plans[q]->prefetch != plans[0]->prefetch)
+            return fail(AWCPML_ERR_ARG, "plans of a batch must share block_y and prefetch (plan %d differs)", q);
+    }
+    awcpml_batch* B = new (std::nothrow) awcpml_batch();
+    if (!B) return fail(AWCPML_ERR_NOMEM, "host allocation failed");
+    B->plans.assign(plans, plans + count);
+    B->ndim = plans[0]->ndim;
+    B->dtype = plans[0]->dtype;
+    B->block_y = plans[0]->block_y;
+    B->prefetch = plans[0]->prefetch;
+    for (auto* P : B->plans) {
+        B->halo_max = std::max(B->halo_max, P->halo);
+        B->nt_max = std::max(B->nt_max, P->nt);
+        B->use32 = B->use32 && (uint64_t)P->field_elems * P->esize < (1ull << 32);
+        B->has_io = B->has_io || P->has_io;
+    }
+    hipError_t e = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete B;
+        return fail(AWCPML_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+    }
+    int st = B->dtype == AWCPML_F64 ? batch_build_t<double>(B) : batch_build_t<float>(B);
+    if (st) {
+        std::string keep = g_err;
+        awcpml_batch_destroy(B);
+        g_err = keep;
+        return st;
+    }
+    *out = B;
+    return AWCPML_OK;
+}
+
+int awcpml_batch_destroy(awcpml_batch* B) {
+    if (!B) return AWCPML_OK;
+    if (B->stream) (void)hipStreamSynchronize(B->stream);
+    drop_graphs(B->graphs);
+    dfree(B->d_args);
+    dfree(B->d_io);
+    if (B->stream) (void)hipStreamDestroy(B->stream);
+    delete B;
+    return AWCPML_OK;
+}
+
+int awcpml_batch_reset(awcpml_batch* B) {
+    g_err.clear();
+    if (!B) return fail(AWCPML_ERR_ARG, "batch is NULL");
+    for (auto* P : B->plans) {
+        int st = awcpml_reset(P);
+        if (st) return st;
+    }
+    B->it = 0;
+    return AWCPML_OK;
+}
+
+int awcpml_batch_advance(awcpml_batch* B, int64_t nsteps) {
+    g_err.clear();
+    if (!B) return fail(AWCPML_ERR_ARG, "batch is NULL");
+    if (nsteps < 0) return fail(AWCPML_ERR_ARG, "nsteps < 0");
+    for (auto* P : B->plans)
+        if (P->it != std::min(B->it, P->nt) || P->io_dirty)
+            return fail(AWCPML_ERR_STATE, "a plan of the batch was stepped or changed on its own; "
+                                          "rebuild the batch or call awcpml_batch_reset");
+    if (B->it + nsteps > B->nt_max)
+        return fail(AWCPML_ERR_STATE, "advance past the longest run: %lld + %lld > %lld",
+                    (long long)B->it, (long long)nsteps, (long long)B->nt_max);
+    int st = batch_run_steps(B, nsteps);
+    if (!st) {
+        hipError_t e = hipStreamSynchronize(B->stream);
+        if (e != hipSuccess) st = fail(AWCPML_ERR_HIP, "hipStreamSynchronize failed: %s", hipGetErrorString(e));
+    }
+    batch_sync_plans(B);
+    return st;
+}
+
+int awcpml_batch_set_graph(awcpml_batch* B, int enable) {
+    g_err.clear();
+    if (!B) return fail(AWCPML_ERR_ARG, "batch is NULL");
+    B->use_graph = enable != 0;
+    if (!B->use_graph) drop_graphs(B->graphs);
+    return AWCPML_OK;
+}
+
+int awcpml_batch_time_advance(awcpml_batch* B, int64_t nsteps, double* ms_total) {
+    g_err.clear();
+    if (!B || !ms_total || nsteps < 0) return fail(AWCPML_ERR_ARG, "bad argument");
+    if (B->it + nsteps > B->nt_max) return fail(AWCPML_ERR_STATE, "advance past the longest run");
+    int st;
+    if (B->use_graph && nsteps > 0 && !find_graph(B->graphs, B->it, nsteps)) {
+        const int64_t it0 = B->it;
+        hipGraphExec_t exec;
+        st = capture_steps(B->stream, nsteps, [&](int64_t q) { return batch_step(B, it0 + q); }, &exec);
+        if (st) return st;
+        B->graphs.push_back({it0, nsteps, exec});
+    }
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, B->stream));
+    st = batch_run_steps(B, nsteps);
+    HIP_TRY(hipEventRecord(e1, B->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_total = ms;
+    batch_sync_plans(B);
+    return st;
+}
+
+int awcpml_batch_time_stencil(awcpml_batch* B, int nlaunches, double* ms_total) {
+    g_err.clear();
+    if (!B || !ms_total || nlaunches < 1) return fail(AWCPML_ERR_ARG, "bad argument");
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, B->stream));
+    int st = AWCPML_OK;
+    for (int q = 0; q < nlaunches && !st; ++q) st = batch_step(B, q % 6, false);
+    HIP_TRY(hipEventRecord(e1, B->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_total = ms;
+    return st;
 }
 
 int awcpml_get_tiling(const awcpml_plan* P, int* block_y, int* march_len, int* prefetch) {

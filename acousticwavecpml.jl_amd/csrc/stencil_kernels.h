@@ -424,12 +424,9 @@ __device__ __forceinline__ void col_slab(const StencilArgs<T>& A, int axis, int6
 // 3D: z-march.  Block (64, BY): lane -> x, threadIdx.y -> y (one row per
 // wave, so y is wave-uniform).  D = ring depth.
 // ---------------------------------------------------------------------
+// One block of a 3D launch: logical tile `lb` of the plan described by A.
 template <typename T, typename Off, int D>
-__global__ __launch_bounds__(256) void stencil3d(const StencilArgs<T> A) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* lds = reinterpret_cast<T*>(smem_raw);
-    const int lb = logical_block(blockIdx.x, A.grid_blocks);
-    if (lb >= A.logical_blocks) return;  // whole block
+__device__ __forceinline__ void stencil3d_block(const StencilArgs<T>& A, int lb, T* lds) {
     if (A.active[2][0] || A.active[2][1]) stage_march_coef(A, 2, lds);
     __syncthreads();
 
@@ -494,11 +491,7 @@ __global__ __launch_bounds__(256) void stencil3d(const StencilArgs<T> A) {
 // 2D: y-march.  Block (64, BY): lane -> x, threadIdx.y -> march chunk.
 // ---------------------------------------------------------------------
 template <typename T, typename Off, int D>
-__global__ __launch_bounds__(256) void stencil2d(const StencilArgs<T> A) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* lds = reinterpret_cast<T*>(smem_raw);
-    const int lb = logical_block(blockIdx.x, A.grid_blocks);
-    if (lb >= A.logical_blocks) return;  // whole block
+__device__ __forceinline__ void stencil2d_block(const StencilArgs<T>& A, int lb, T* lds) {
     if (A.active[1][0] || A.active[1][1]) stage_march_coef(A, 1, lds);
     __syncthreads();
 
@@ -549,32 +542,119 @@ __global__ __launch_bounds__(256) void stencil2d(const StencilArgs<T> A) {
 }
 
 // ---------------------------------------------------------------------
+// Entry kernels.  A plain launch covers one plan; a batch launch covers up
+// to kMaxBatch plans (different grids, same precision) in one grid: block b
+// (after the XCD renumbering) belongs to the plan p with
+// block_start[p] <= b < block_start[p+1].  A plan whose run is shorter than
+// the batch's is skipped once it0 reaches its nt.
+// ---------------------------------------------------------------------
+constexpr int kMaxBatch = 32;
+
+template <typename T>
+struct BatchArgs {
+    const StencilArgs<T>* table;      // per-plan arguments for this step's rotation
+    int64_t it0;                      // 0-based step index
+    int64_t nt[kMaxBatch];
+    int block_start[kMaxBatch + 1];
+    int nplans;
+    int grid_blocks;
+};
+
+__device__ __forceinline__ int batch_plan(const int* start, int nplans, int lb) {
+    int p = 0;
+    for (int q = 1; q < nplans; ++q)
+        if (lb >= start[q]) p = q;
+    return p;
+}
+
+template <typename T, typename Off, int D>
+__global__ __launch_bounds__(256) void stencil3d(const StencilArgs<T> A) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int lb = logical_block(blockIdx.x, A.grid_blocks);
+    if (lb >= A.logical_blocks) return;  // whole block
+    stencil3d_block<T, Off, D>(A, lb, reinterpret_cast<T*>(smem_raw));
+}
+
+template <typename T, typename Off, int D>
+__global__ __launch_bounds__(256) void stencil2d(const StencilArgs<T> A) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int lb = logical_block(blockIdx.x, A.grid_blocks);
+    if (lb >= A.logical_blocks) return;  // whole block
+    stencil2d_block<T, Off, D>(A, lb, reinterpret_cast<T*>(smem_raw));
+}
+
+template <typename T, typename Off, int D, int NDIM>
+__global__ __launch_bounds__(256) void stencil_batch(const BatchArgs<T> B) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int lb = logical_block(blockIdx.x, B.grid_blocks);
+    if (lb >= B.block_start[B.nplans]) return;  // whole block
+    const int p = batch_plan(B.block_start, B.nplans, lb);
+    if (B.it0 >= B.nt[p]) return;
+    const StencilArgs<T>& A = B.table[p];
+    if (NDIM == 3)
+        stencil3d_block<T, Off, D>(A, lb - B.block_start[p], reinterpret_cast<T*>(smem_raw));
+    else
+        stencil2d_block<T, Off, D>(A, lb - B.block_start[p], reinterpret_cast<T*>(smem_raw));
+}
+
+// ---------------------------------------------------------------------
 // Sources and receivers of step `it0` (0-based), one block per shot:
 // inject_sources! then record_receivers! (acoustic_2D_xPU.jl:77-91,126-128).
 // Sources of a shot that share a node are added by one thread in source
 // order (seq[shot] = 1), otherwise one thread per source.
 // ---------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void sources_receivers(
-    T* __restrict__ pnew, int64_t field_stride,
-    const int64_t* __restrict__ src_lin, const T* __restrict__ src_wav,
-    const int* __restrict__ src_start, const int* __restrict__ src_seq,
-    const int64_t* __restrict__ rec_lin, T* __restrict__ traces,
-    const int* __restrict__ rec_start, int64_t nt, int64_t it0) {
-    const int shot = blockIdx.x;
-    T* p = pnew + (int64_t)shot * field_stride;
-    const int s0 = src_start[shot], s1 = src_start[shot + 1];
-    if (src_seq[shot]) {
+struct IoArgs {
+    T* pnew;                 // this step's pnew (shot 0)
+    int64_t field_stride;
+    const int64_t* src_lin;  // element offsets of the sources, all shots
+    const T* src_wav;        // nt values per source
+    const int* src_start;    // nshots+1 CSR offsets
+    const int* src_seq;      // 1: the shot has sources sharing a node
+    const int64_t* rec_lin;
+    T* traces;               // nt values per receiver
+    const int* rec_start;
+    int64_t nt;
+};
+
+template <typename T>
+__device__ __forceinline__ void io_block(const IoArgs<T>& A, int shot, int64_t it0) {
+    T* p = A.pnew + (int64_t)shot * A.field_stride;
+    const int s0 = A.src_start[shot], s1 = A.src_start[shot + 1];
+    if (A.src_seq[shot]) {
         if (threadIdx.x == 0)
-            for (int e = s0; e < s1; ++e) p[src_lin[e]] += src_wav[(int64_t)e * nt + it0];
+            for (int e = s0; e < s1; ++e) p[A.src_lin[e]] += A.src_wav[(int64_t)e * A.nt + it0];
     } else {
         for (int e = s0 + threadIdx.x; e < s1; e += blockDim.x)
-            p[src_lin[e]] += src_wav[(int64_t)e * nt + it0];
+            p[A.src_lin[e]] += A.src_wav[(int64_t)e * A.nt + it0];
     }
     __syncthreads();
-    const int r0 = rec_start[shot], r1 = rec_start[shot + 1];
+    const int r0 = A.rec_start[shot], r1 = A.rec_start[shot + 1];
     for (int r = r0 + threadIdx.x; r < r1; r += blockDim.x)
-        traces[(int64_t)r * nt + it0] = p[rec_lin[r]];
+        A.traces[(int64_t)r * A.nt + it0] = p[A.rec_lin[r]];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void sources_receivers(const IoArgs<T> A, int64_t it0) {
+    io_block(A, blockIdx.x, it0);
+}
+
+// Batch: block b serves shot b - shot_start[p] of plan p.
+template <typename T>
+struct IoBatchArgs {
+    const IoArgs<T>* table;   // per-plan arguments for this step's rotation
+    int64_t it0;
+    int shot_start[kMaxBatch + 1];
+    int nplans;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void sources_receivers_batch(const IoBatchArgs<T> B) {
+    const int b = blockIdx.x;
+    const int p = batch_plan(B.shot_start, B.nplans, b);
+    const IoArgs<T>& A = B.table[p];
+    if (B.it0 >= A.nt) return;
+    io_block(A, b - B.shot_start[p], B.it0);
 }
 
 }  // namespace awcpml

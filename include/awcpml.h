@@ -146,6 +146,41 @@ int awcpml_set_tiling(awcpml_plan *plan, int block_y, int march_len, int prefetc
 int awcpml_get_tiling(const awcpml_plan *plan, int *block_y, int *march_len,
                       int *prefetch);
 
+/* Capture the time loop in HIP graphs: with enable != 0, awcpml_advance and
+ * awcpml_time_advance record each (first step, number of steps) sequence
+ * of launches once and replay it afterwards, so a solve after
+ * awcpml_reset costs one graph launch instead of two launches per step.
+ * No counterpart in the reference (its loop launches every kernel from
+ * Julia, acoustic_2D_xPU.jl:327-345). */
+int awcpml_set_graph(awcpml_plan *plan, int enable);
+
+/* ------------------------------------------------------------------ */
+/* Batches: several plans advanced by one launch per step.             */
+/* ------------------------------------------------------------------ */
+
+/* Group 1..32 plans that share ndim, dtype and tiling (grids, halos,
+ * models, shot counts and nt may all differ) so that every time step of
+ * all of them is one stencil launch and one source/receiver launch.  The
+ * plans stay owned by the caller and must be fully configured; a plan whose
+ * nt is shorter than the batch's longest stops at its own nt.  Results are
+ * read per plan with awcpml_get_traces / awcpml_get_field.  Extension
+ * beyond the reference, which solves one model per call
+ * (src/shared/acoustic_3D_xPU.jl:169). */
+typedef struct awcpml_batch awcpml_batch;
+int awcpml_batch_create(awcpml_plan *const *plans, int count, awcpml_batch **out);
+int awcpml_batch_destroy(awcpml_batch *batch);
+/* awcpml_reset on every plan; the batch starts again at step 0. */
+int awcpml_batch_reset(awcpml_batch *batch);
+/* Run the next nsteps steps of every plan (each stops at its own nt). */
+int awcpml_batch_advance(awcpml_batch *batch, int64_t nsteps);
+/* Graph capture of the batch's step sequences (see awcpml_set_graph). */
+int awcpml_batch_set_graph(awcpml_batch *batch, int enable);
+/* awcpml_batch_advance timed with HIP events on the batch's stream (ms). */
+int awcpml_batch_time_advance(awcpml_batch *batch, int64_t nsteps, double *ms_total);
+/* Time nlaunches batch stencil launches alone (ms); the plans' wavefields
+ * advance without sources; call awcpml_batch_reset afterwards. */
+int awcpml_batch_time_stencil(awcpml_batch *batch, int nlaunches, double *ms_total);
+
 /* Message of the calling thread's last failed call ("" if none). */
 const char *awcpml_last_error(void);
 
