@@ -83,6 +83,10 @@ struct StencilArgs {
     int tiles_x, tiles_y, nchunks, march_len, nshots;
     int logical_blocks;
     int grid_blocks;          // launched blocks, a multiple of kNumXcd
+    // optional: per node the derivative of pnew with respect to fact,
+    // q = sum d2 + sum over slabs (dpsi + xi), written for the adjoint's
+    // gradient (same layout as the fields, one array per shot); null = off
+    T* qout;
 };
 
 __device__ __forceinline__ int logical_block(int b, int grid) {
@@ -142,9 +146,9 @@ __device__ __forceinline__ LayerCoef<T> layer_coef(const CpmlSide<T>& s, int m) 
 // (already loaded).  pm, pc, pp: pressure before, at and after the node
 // along the axis; d2: the node's second derivative along the axis; f: fact.
 // Stores the new psi[m] (and psi[m-1] when m == 1) and xi[m-1]; returns
-// fact*(dpsi + xi), the reference's damping term
-// (acoustic_2D_xPU.jl:7-18 and 45-56 for x-left; the other sides are
-// identical up to index shifts).
+// dpsi + xi, which the caller scales by fact into the reference's damping
+// term damp += fact*(dpsi + xi) (acoustic_2D_xPU.jl:7-18 and 45-56 for
+// x-left; the other sides are identical up to index shifts).
 template <typename T, typename Off>
 __device__ __forceinline__ T cpml_apply(const LayerCoef<T>& c, T* psi_wr, T* xi_arr, Off psi_m,
                                         Off psi_m1, Off xi_m1, int m, T psi_old,
@@ -159,7 +163,7 @@ __device__ __forceinline__ T cpml_apply(const LayerCoef<T>& c, T* psi_wr, T* xi_
     const T dpsi = (psi_here - psi_prev) * inv_d;
     const T xin = c.bK1 * xi_old + c.a1 * (d2 + dpsi);
     stg(xi_arr, xi_m1, xin);
-    return f * (dpsi + xin);
+    return dpsi + xin;
 }
 
 // Pointers of one slab side for one shot, selected without memory loads.
@@ -256,6 +260,7 @@ struct March {
     int mshift[2];            // march layer = node - mshift[side]
     const T* lds;             // march-axis coefficients
     int h1;
+    T* qout;                  // optional d(pnew)/d(fact) output (this shot)
 };
 
 template <typename T, typename Off, int NDIM, bool SA, bool SB, bool SM>
@@ -306,30 +311,39 @@ __device__ __forceinline__ void compute_node(const March<T, Off, NDIM>& M, int k
     if (NDIM == 3) d2b = (q.yp - two * pc + q.ym) * M.i2b;
     const T d2m = (pp - two * pc + pm) * M.i2m;
     T damp = T(0);
+    T slab = T(0);  // sum of (dpsi + xi) over the node's slabs, for qout
     if (SA) {
-        if (M.sa.in)
-            damp += cpml_apply(M.sa.c, M.sa.p.psi_wr, M.sa.p.xi, M.sa.o_psi + (Off)k * M.sa.st_psi,
-                               M.sa.o_psi1 + (Off)k * M.sa.st_psi, M.sa.o_xi + (Off)k * M.sa.st_xi,
-                               M.sa.m, q.psa0, q.psa1, q.xia, q.xm, pc, q.xp, d2a, M.ia, f);
+        if (M.sa.in) {
+            const T t = cpml_apply(M.sa.c, M.sa.p.psi_wr, M.sa.p.xi, M.sa.o_psi + (Off)k * M.sa.st_psi,
+                                   M.sa.o_psi1 + (Off)k * M.sa.st_psi, M.sa.o_xi + (Off)k * M.sa.st_xi,
+                                   M.sa.m, q.psa0, q.psa1, q.xia, q.xm, pc, q.xp, d2a, M.ia, f);
+            damp += f * t;
+            slab += t;
+        }
     }
     if (NDIM == 3 && SB) {
-        damp += cpml_apply(M.sb.c, M.sb.p.psi_wr, M.sb.p.xi, M.sb.o_psi + (Off)k * M.sb.st_psi,
-                           M.sb.o_psi1 + (Off)k * M.sb.st_psi, M.sb.o_xi + (Off)k * M.sb.st_xi,
-                           M.sb.m, q.psb0, q.psb1, q.xib, q.ym, pc, q.yp, d2b, M.ib, f);
+        const T t = cpml_apply(M.sb.c, M.sb.p.psi_wr, M.sb.p.xi, M.sb.o_psi + (Off)k * M.sb.st_psi,
+                               M.sb.o_psi1 + (Off)k * M.sb.st_psi, M.sb.o_xi + (Off)k * M.sb.st_xi,
+                               M.sb.m, q.psb0, q.psb1, q.xib, q.ym, pc, q.yp, d2b, M.ib, f);
+        damp += f * t;
+        slab += t;
     }
     if (SM) {
         const int mz = k - M.mshift[mside];
         const LayerCoef<T> cz = lds_coef(M.lds, M.h1, mside, mz);
         const SlabPtrs<T>& P = M.pm_side[mside];
-        damp += cpml_apply(cz, P.psi_wr, P.xi, M.col + (Off)mz * M.sm,
-                           M.col + (Off)(mz - 1) * M.sm, M.col + (Off)(mz - 1) * M.sm, mz,
-                           q.psm0, q.psm1, q.xim, pm, pc, pp, d2m, M.imm, f);
+        const T t = cpml_apply(cz, P.psi_wr, P.xi, M.col + (Off)mz * M.sm,
+                               M.col + (Off)(mz - 1) * M.sm, M.col + (Off)(mz - 1) * M.sm, mz,
+                               q.psm0, q.psm1, q.xim, pm, pc, pp, d2m, M.imm, f);
+        damp += f * t;
+        slab += t;
     }
     // update_p!: pnew = 2 pcur - pold + fact*(d2x + d2y (+ d2z)) + damp
     T lap;
     if (NDIM == 3) lap = d2a + d2b + d2m;   // (d2x + d2y) + d2z
     else lap = d2a + d2m;                   // d2x + d2y
     stg(M.pnew, M.col + (Off)k * M.sm, two * pc - q.po + f * lap + damp);
+    if (M.qout) stg(M.qout, M.col + (Off)k * M.sm, lap + slab);
 }
 
 // March over nodes [k_from, k_to) with a fixed load set.  Each node's slot
@@ -453,6 +467,7 @@ __device__ __forceinline__ void stencil3d_block(const StencilArgs<T>& A, int lb,
     M.pold = A.pold + fo;
     M.pnew = A.pnew + fo;
     M.fact = A.fact;
+    M.qout = A.qout ? A.qout + fo : nullptr;
     M.sy = (Off)A.px * E;
     M.sm = (Off)(A.px * ny) * E;
     M.col = ((Off)i + (Off)A.px * (Off)j) * E;
@@ -516,6 +531,7 @@ __device__ __forceinline__ void stencil2d_block(const StencilArgs<T>& A, int lb,
     M.pold = A.pold + fo;
     M.pnew = A.pnew + fo;
     M.fact = A.fact;
+    M.qout = A.qout ? A.qout + fo : nullptr;
     M.sy = (Off)A.px * E;
     M.sm = M.sy;
     M.col = (Off)i * E;

@@ -22,6 +22,7 @@
  *   forward! + time loop . src/shared/acoustic_2D_xPU.jl:93-131,327-345,
  *                          3D:124-167,392-407 (buffer rotation)
  *   outputs .............. src/shared/acoustic_2D_xPU.jl:441,444; 3D:541,544
+ *   FWI gradient ......... discrete adjoint of the loop above (oracle_fwi_gradient)
  *
  * Parallelism mirrors ParallelStencil's Threads backend: the outermost grid
  * index is split over OpenMP threads, x innermost.  Results do not depend
@@ -600,6 +601,245 @@ long oracle_run(oracle_t *o, long nsteps)
         ++done;
     }
     return done;
+}
+
+/* ------------------------------------------------------------------ */
+/* discrete adjoint (FWI gradient), FP64, 2D and 3D                   */
+/* ------------------------------------------------------------------ */
+/*
+ * The reference has no gradient ("WIP", README.md:297).  This is the exact
+ * reverse-mode transpose of the time step above, written operation by
+ * operation in scatter form, for the misfit J = 1/2 sum_{it,r}
+ * (trace[it,r] - obs[it,r])^2.  It returns dJ/dfact at every node with dt
+ * and the CPML coefficients held fixed (fact = dt^2 vel^2, so
+ * dJ/dvel = dJ/dfact * 2 dt^2 vel).  Dirichlet nodes never depend on the
+ * state or on fact (they only accumulate sources), so the transpose runs
+ * over interior nodes only.  The whole forward history is kept in memory:
+ * meant for the small grids of the gradient checks.
+ */
+
+/* zero state and traces, step counter back to 0 */
+void oracle_reset(oracle_t *o)
+{
+    size_t N = (size_t)o->n[0] * o->n[1] * o->n[2];
+    for (int b = 0; b < 3; ++b) memset(o->p[b], 0, sizeof(double) * N);
+    for (int a = 0; a < o->ndim; ++a)
+        for (int s = 0; s < 2; ++s) {
+            memset(o->psi[a][s], 0, sizeof(double) * slab_size(o, a, o->halo + 1));
+            memset(o->xi[a][s], 0, sizeof(double) * slab_size(o, a, o->halo));
+        }
+    memset(o->traces, 0, sizeof(double) * (size_t)o->nt * o->nrecs);
+    o->it_done = 0;
+}
+
+/* replace fact (e.g. for finite-difference checks with dt fixed) */
+void oracle_set_fact(oracle_t *o, const double *fact)
+{
+    size_t N = (size_t)o->n[0] * o->n[1] * o->n[2];
+    memcpy(o->fact, fact, sizeof(double) * N);
+}
+
+/* linear index of node c with coordinate `axis` replaced by `layer` in an
+ * array whose extent along `axis` is `layers` */
+static size_t slab_lin(const oracle_t *o, int axis, const long *c, long layer, long layers)
+{
+    size_t idx = 0, stride = 1;
+    for (int a = 0; a < o->ndim; ++a) {
+        long v = (a == axis) ? layer : c[a];
+        long ext = (a == axis) ? layers : o->n[a];
+        idx += (size_t)v * stride;
+        stride *= (size_t)ext;
+    }
+    return idx;
+}
+
+/*
+ * FWI gradient by the discrete adjoint of the time loop above (reverse-mode
+ * transpose of every statement, scatter form, full forward history kept in
+ * memory -- small cases only).
+ *
+ *   J(fact) = 1/2 sum_{it,r} (traces[it,r] - obs[it,r])^2     (obs: nt x nrecs)
+ *   grad[i] = dJ/dfact[i] with dt, the CPML coefficients and the source
+ *             wavelets held fixed (they depend on vmax, not on each node).
+ *
+ * The reference has no gradient yet (README.md:297 lists the adjoint and
+ * checkpointing as work in progress); this is the exact gradient of the
+ * reference's discrete forward map restated above.  dJ/dvel = grad * 2 dt^2
+ * vel.  Pinned by central finite differences of oracle_run's misfit
+ * (tests/test_oracle_adjoint.py).
+ * Returns 0, or -1 when the history does not fit in memory.
+ */
+int oracle_fwi_gradient(oracle_t *o, const double *obs, double *grad, double *misfit)
+{
+    const int nd = o->ndim;
+    const long h = o->halo, hp1 = h + 1;
+    const long nt = o->nt;
+    const size_t N = (size_t)o->n[0] * o->n[1] * o->n[2];
+    size_t stride[3] = {1, (size_t)o->n[0], (size_t)o->n[0] * o->n[1]};
+    size_t psz[3] = {0, 0, 0}, xsz[3] = {0, 0, 0};
+    for (int a = 0; a < nd; ++a) {
+        psz[a] = slab_size(o, a, hp1);
+        xsz[a] = slab_size(o, a, h);
+    }
+    /* forward with history: pcur at the start of each step, psi/xi after */
+    double *Bh = (double *)malloc(sizeof(double) * N * (size_t)(nt > 0 ? nt : 1));
+    double *PSh[3][2] = {{0}}, *XIh[3][2] = {{0}};
+    int ok = Bh != NULL;
+    for (int a = 0; a < nd && ok; ++a)
+        for (int s = 0; s < 2 && ok; ++s) {
+            PSh[a][s] = (double *)malloc(sizeof(double) * psz[a] * (size_t)(nt > 0 ? nt : 1));
+            XIh[a][s] = (double *)malloc(sizeof(double) * (xsz[a] ? xsz[a] : 1) * (size_t)(nt > 0 ? nt : 1));
+            ok = PSh[a][s] && XIh[a][s];
+        }
+    double *lamA = (double *)calloc(N, sizeof(double));
+    double *lamB = (double *)calloc(N, sizeof(double));
+    double *lamC = (double *)calloc(N, sizeof(double));
+    double *lamAn = (double *)calloc(N, sizeof(double));
+    double *lamBn = (double *)calloc(N, sizeof(double));
+    double *lP[3][2] = {{0}}, *lPn[3][2] = {{0}}, *lX[3][2] = {{0}}, *lXn[3][2] = {{0}};
+    ok = ok && lamA && lamB && lamC && lamAn && lamBn;
+    for (int a = 0; a < nd && ok; ++a)
+        for (int s = 0; s < 2 && ok; ++s) {
+            lP[a][s] = zalloc(psz[a]);
+            lPn[a][s] = zalloc(psz[a]);
+            lX[a][s] = zalloc(xsz[a]);
+            lXn[a][s] = zalloc(xsz[a]);
+            ok = lP[a][s] && lPn[a][s] && lX[a][s] && lXn[a][s];
+        }
+    int status = ok ? ORACLE_OK : ORACLE_ERR_NOMEM;
+    if (!ok) goto done;
+
+    oracle_reset(o);
+    for (long it = 1; it <= nt; ++it) {
+        memcpy(Bh + (size_t)(it - 1) * N, o->p[1], sizeof(double) * N);
+        oracle_step(o, it);
+        for (int a = 0; a < nd; ++a)
+            for (int s = 0; s < 2; ++s) {
+                memcpy(PSh[a][s] + (size_t)(it - 1) * psz[a], o->psi[a][s], sizeof(double) * psz[a]);
+                if (xsz[a])
+                    memcpy(XIh[a][s] + (size_t)(it - 1) * xsz[a], o->xi[a][s], sizeof(double) * xsz[a]);
+            }
+    }
+    double J = 0.0;
+    for (size_t q = 0; q < (size_t)nt * o->nrecs; ++q) {
+        double r = o->traces[q] - obs[q];
+        J += 0.5 * r * r;
+    }
+    if (misfit) *misfit = J;
+    memset(grad, 0, sizeof(double) * N);
+
+    for (long it = nt; it >= 1; --it) {
+        const double *B = Bh + (size_t)(it - 1) * N;
+        /* adjoint of this step's output C: from the next step's pcur slot
+         * plus the receiver residuals (record_receivers! transposed) */
+        memcpy(lamC, lamBn, sizeof(double) * N);
+        for (int r = 0; r < o->nrecs; ++r) {
+            size_t idx = 0;
+            for (int a = 0; a < nd; ++a) idx += (size_t)(o->posrecs[(size_t)a * o->nrecs + r] - 1) * stride[a];
+            lamC[idx] += o->traces[(size_t)r * nt + (it - 1)] - obs[(size_t)r * nt + (it - 1)];
+        }
+        /* B is passed on as the next step's pold */
+        memcpy(lamB, lamAn, sizeof(double) * N);
+        memset(lamA, 0, sizeof(double) * N);
+        for (int a = 0; a < nd; ++a)
+            for (int s = 0; s < 2; ++s) {
+                memset(lPn[a][s], 0, sizeof(double) * psz[a]);
+                if (xsz[a]) memset(lXn[a][s], 0, sizeof(double) * xsz[a]);
+            }
+        /* update_p! transposed */
+        long c[3] = {0, 0, 0};
+        const long lo[3] = {1, 1, nd == 3 ? 1 : 0};
+        const long hi[3] = {o->n[0] - 1, o->n[1] - 1, nd == 3 ? o->n[2] - 1 : 1};
+        for (c[2] = lo[2]; c[2] < hi[2]; ++c[2])
+            for (c[1] = lo[1]; c[1] < hi[1]; ++c[1])
+                for (c[0] = lo[0]; c[0] < hi[0]; ++c[0]) {
+                    size_t x = (size_t)c[0] + (size_t)c[1] * stride[1] + (size_t)c[2] * stride[2];
+                    const double lc = lamC[x];
+                    const double f = o->fact[x];
+                    lamA[x] -= lc;
+                    lamB[x] += 2.0 * lc;
+                    double qsum = 0.0;
+                    for (int a = 0; a < nd; ++a) {
+                        const double d2 = (B[x + stride[a]] - 2.0 * B[x] + B[x - stride[a]]) * o->inv_d2[a];
+                        qsum += d2;
+                        double lam_d2 = f * lc;
+                        long q = c[a], m = -1;
+                        int side = -1;
+                        if (q <= h) { side = 0; m = q; }
+                        else if (q >= o->n[a] - h - 1) { side = 1; m = q - (o->n[a] - h - 2); }
+                        if (side >= 0) {
+                            const double *Ps = PSh[a][side] + (size_t)(it - 1) * psz[a];
+                            const double *Xs = XIh[a][side] + (size_t)(it - 1) * xsz[a];
+                            const size_t pm = slab_lin(o, a, c, m, hp1), pm1 = slab_lin(o, a, c, m - 1, hp1);
+                            const size_t xm1 = slab_lin(o, a, c, m - 1, h);
+                            const double dpsi = (Ps[pm] - Ps[pm1]) * o->inv_d[a];
+                            qsum += dpsi + Xs[xm1];
+                            const double bK = o->coef[a][2 + side][m - 1], aa = o->coef[a][side][m - 1];
+                            /* xi' is read by damp (f) and by the next step */
+                            const double lxi = lX[a][side][xm1] + f * lc;
+                            lXn[a][side][xm1] += bK * lxi;
+                            lam_d2 += aa * lxi;
+                            const double ldpsi = f * lc + aa * lxi;
+                            lP[a][side][pm] += ldpsi * o->inv_d[a];
+                            lP[a][side][pm1] -= ldpsi * o->inv_d[a];
+                        }
+                        lamB[x + stride[a]] += lam_d2 * o->inv_d2[a];
+                        lamB[x] -= 2.0 * lam_d2 * o->inv_d2[a];
+                        lamB[x - stride[a]] += lam_d2 * o->inv_d2[a];
+                    }
+                    grad[x] += lc * qsum;
+                }
+        /* update_psi_*! transposed (all layers, all rows, as launched) */
+        if (h > 0)
+            for (int a = 0; a < nd; ++a)
+                for (int s = 0; s < 2; ++s) {
+                    const double *ah = o->coef[a][4 + s], *bKh = o->coef[a][6 + s];
+                    const long shift = s ? o->n[a] - h - 2 : 0;
+                    long e[3] = {0, 0, 0};
+                    long ext[3] = {o->n[0], o->n[1], nd == 3 ? o->n[2] : 1};
+                    ext[a] = hp1;
+                    for (e[2] = 0; e[2] < ext[2]; ++e[2])
+                        for (e[1] = 0; e[1] < ext[1]; ++e[1])
+                            for (e[0] = 0; e[0] < ext[0]; ++e[0]) {
+                                const long m = e[a];
+                                const size_t pi = slab_lin(o, a, e, m, hp1);
+                                const double lp = lP[a][s][pi];
+                                if (lp == 0.0) continue;
+                                lPn[a][s][pi] += bKh[m] * lp;
+                                long nc[3] = {e[0], e[1], e[2]};
+                                nc[a] = m + shift;
+                                size_t k = 0;
+                                for (int b = 0; b < nd; ++b) k += (size_t)nc[b] * stride[b];
+                                lamB[k + stride[a]] += ah[m] * lp * o->inv_d[a];
+                                lamB[k] -= ah[m] * lp * o->inv_d[a];
+                            }
+                }
+        memcpy(lamAn, lamA, sizeof(double) * N);
+        memcpy(lamBn, lamB, sizeof(double) * N);
+        for (int a = 0; a < nd; ++a)
+            for (int s = 0; s < 2; ++s) {
+                memcpy(lP[a][s], lPn[a][s], sizeof(double) * psz[a]);
+                if (xsz[a]) memcpy(lX[a][s], lXn[a][s], sizeof(double) * xsz[a]);
+            }
+    }
+
+done:
+    free(Bh);
+    free(lamA);
+    free(lamB);
+    free(lamC);
+    free(lamAn);
+    free(lamBn);
+    for (int a = 0; a < 3; ++a)
+        for (int s = 0; s < 2; ++s) {
+            free(PSh[a][s]);
+            free(XIh[a][s]);
+            free(lP[a][s]);
+            free(lPn[a][s]);
+            free(lX[a][s]);
+            free(lXn[a][s]);
+        }
+    return status;
 }
 
 /* ------------------------------------------------------------------ */

@@ -81,6 +81,10 @@ def lib():
                                             ctypes.c_double, ctypes.c_double] + [_dp] * 13)
         L.oracle_update_psi_3d.argtypes = ([ctypes.c_long] * 3 + [ctypes.c_int]
                                            + [ctypes.c_double] * 3 + [_dp] * 19)
+        L.oracle_reset.argtypes = [ctypes.c_void_p]
+        L.oracle_set_fact.argtypes = [ctypes.c_void_p, _dp]
+        L.oracle_fwi_gradient.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.oracle_fwi_gradient.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -181,6 +185,35 @@ class OracleProblem:
 
     def __del__(self):
         self.close()
+
+    def reset(self):
+        lib().oracle_reset(self._h)
+
+    def set_fact(self, fact):
+        f = np.asfortranarray(np.asarray(fact, dtype=np.float64))
+        assert f.shape == self.n
+        lib().oracle_set_fact(self._h, f.ctypes.data_as(_dp))
+
+    def misfit(self, obs):
+        """Reset, run all steps, return 1/2 sum (traces - obs)^2."""
+        self.reset()
+        self.run()
+        r = self.traces() - obs
+        return 0.5 * float((r * r).sum())
+
+    def fwi_gradient(self, obs):
+        """(misfit, dJ/dfact) of the L2 trace misfit by the discrete adjoint
+        (dt and the CPML coefficients held fixed).  Leaves the forward run's
+        traces and fields in place."""
+        obs = np.asfortranarray(np.asarray(obs, dtype=np.float64))
+        assert obs.shape == (self.nt, self.nrecs)
+        g = np.zeros(self.n, order="F")
+        J = ctypes.c_double()
+        st = lib().oracle_fwi_gradient(self._h, obs.ctypes.data_as(_dp), g.ctypes.data_as(_dp),
+                                       ctypes.byref(J))
+        if st != 0:
+            raise ValueError(ERRORS.get(st, st))
+        return J.value, g
 
     def run(self, nsteps=None):
         if nsteps is None:
