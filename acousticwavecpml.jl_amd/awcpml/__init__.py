@@ -6,11 +6,14 @@ Public surface (mirrors the reference's exports and solver entry points):
   Plan                  the C-ABI plan (multi-shot, FP64/FP32, graph capture)
   Batch                 several plans (uneven grids) advanced by one launch per step
   autotune              launch-shape autotuner with a JSON cache
+  fwi                   FWI gradients through the GPU adjoint (misfit_gradient,
+                        velocity_gradient, solve_traces: a torch.autograd.Function)
 """
 from .host import Receivers, Sources, calc_Kab_CPML, rickersource1D, setup  # noqa: F401
 from .solver import Batch, Plan, solve2D, solve3D  # noqa: F401
 from .autotune import autotune  # noqa: F401
 from . import _lib  # noqa: F401
+from . import fwi  # noqa: F401
 
 __all__ = ["Sources", "Receivers", "rickersource1D", "calc_Kab_CPML", "setup", "Plan",
-           "Batch", "autotune", "solve2D", "solve3D"]
+           "Batch", "autotune", "solve2D", "solve3D", "fwi"]

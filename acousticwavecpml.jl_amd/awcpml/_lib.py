@@ -56,6 +56,8 @@ SIGNATURES = {
     "awcpml_batch_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
     "awcpml_batch_time_advance": (ctypes.c_int, [_vp, ctypes.c_int64, _dp]),
     "awcpml_batch_time_stencil": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "awcpml_checkpoint_run": (ctypes.c_int, [_vp, ctypes.c_int64]),
+    "awcpml_adjoint": (ctypes.c_int, [_vp, _dp, ctypes.c_int64, _dp]),
     "awcpml_last_error": (ctypes.c_char_p, []),
     "awcpml_version": (ctypes.c_char_p, []),
 }

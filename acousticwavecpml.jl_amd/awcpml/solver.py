@@ -133,6 +133,33 @@ class Plan:
         _lib.check(_lib.load().awcpml_get_traces(self._h, shot, out.ctypes.data_as(_lib._dp)))
         return out
 
+    # -- differentiation ----------------------------------------------
+    def checkpoint_run(self, every=0):
+        """Reset, run all nt steps and keep a forward checkpoint every
+        ``every`` steps (0: round(sqrt(nt))) for :meth:`adjoint`."""
+        _lib.check(_lib.load().awcpml_checkpoint_run(self._h, int(every)))
+
+    def adjoint(self, trace_bar):
+        """dJ/dfact summed over the shots, for the trace cotangents
+        ``trace_bar`` = dJ/dtraces: one (nt, nrecs) array per shot (a list,
+        or a single array for a one-shot plan).  Needs a preceding
+        :meth:`checkpoint_run` on the same model and acquisition."""
+        if isinstance(trace_bar, np.ndarray) and trace_bar.ndim == 2:
+            trace_bar = [trace_bar]
+        if len(trace_bar) != self.nshots:
+            raise ValueError(f"need {self.nshots} cotangent arrays, got {len(trace_bar)}")
+        parts = []
+        for s, t in enumerate(trace_bar):
+            t = np.asarray(t, dtype=np.float64)
+            if t.shape != (self.nt, self._nrecs[s]):
+                raise ValueError(f"shot {s}: cotangent shape {t.shape} != {(self.nt, self._nrecs[s])}")
+            parts.append(t.ravel(order="F"))
+        flat = np.ascontiguousarray(np.concatenate(parts)) if parts else np.zeros(1)
+        out = np.zeros(self.n, order="F")
+        _lib.check(_lib.load().awcpml_adjoint(self._h, _lib.dptr(flat), flat.size if parts else 0,
+                                              out.ctypes.data_as(_lib._dp)))
+        return out
+
     def field(self, shot=0, which=1):
         out = np.zeros(self.n, order="F")
         _lib.check(_lib.load().awcpml_get_field(self._h, shot, which, out.ctypes.data_as(_lib._dp)))

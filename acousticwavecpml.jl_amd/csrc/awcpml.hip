@@ -17,12 +17,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
 
+#include "adjoint_kernels.h"
 #include "awcpml.h"
 #include "stencil_kernels.h"
 
@@ -109,6 +111,17 @@ struct awcpml_plan {
     int block_y = 0, march_len = 0;
     int prefetch = 2;  // software-pipeline depth of the march (1..3)
     int device = 0;
+
+    // adjoint: forward-state checkpoints every ck_every steps (state after
+    // c*ck_every steps: the three field buffers, psi of the current parity,
+    // xi), valid until the model, CPML or sources/receivers change
+    void* ck = nullptr;
+    size_t ck_bytes = 0;  // per checkpoint
+    int64_t ck_every = 0, ck_count = 0;
+    size_t ck_alloc = 0;
+    bool ck_valid = false;
+    std::vector<int> ck_rot, ck_par;
+    void* q_cur = nullptr;  // d(pnew)/d(fact) output of the next stencil launch
 };
 
 namespace {
@@ -186,6 +199,7 @@ StencilArgs<T> make_args(const awcpml_plan* P, int rot, int par) {
     A.pcur = (const T*)P->p[(rot + 1) % 3];
     A.pnew = (T*)P->p[(rot + 2) % 3];
     A.fact = (const T*)P->fact;
+    A.qout = (T*)P->q_cur;
     for (int a = 0; a < 3; ++a) {
         A.inv_d[a] = (T)P->inv_d[a];
         A.inv_d2[a] = (T)P->inv_d2[a];
@@ -680,6 +694,7 @@ int awcpml_plan_destroy(awcpml_plan* P) {
     dfree(P->d_rec_lin);
     dfree(P->d_rec_start);
     dfree(P->d_traces);
+    dfree(P->ck);
     if (P->stream) (void)hipStreamDestroy(P->stream);
     delete P;
     return AWCPML_OK;
@@ -690,6 +705,7 @@ int awcpml_set_model(awcpml_plan* P, const double* fact, const double* inv_d, co
     int st = check_plan(P);
     if (st) return st;
     if (!fact || !inv_d || !inv_d2) return fail(AWCPML_ERR_ARG, "NULL argument");
+    P->ck_valid = false;
     for (int a = 0; a < P->ndim; ++a) {
         P->inv_d[a] = inv_d[a];
         P->inv_d2[a] = inv_d2[a];
@@ -731,6 +747,7 @@ int awcpml_set_cpml(awcpml_plan* P, int axis, const double* a_l, const double* a
         P->active[axis][s] = (h > 0 && !idle) ? 1 : 0;
     }
     P->cpml_set[axis] = true;
+    P->ck_valid = false;
     drop_graphs(P->graphs);
     return AWCPML_OK;
 }
@@ -758,6 +775,7 @@ int awcpml_set_sources(awcpml_plan* P, int shot, int nsrcs, const int64_t* pos, 
     P->src_wav[shot].assign(dt2srctf, dt2srctf + (size_t)nt * nsrcs);
     P->src_set[shot] = true;
     P->io_dirty = true;
+    P->ck_valid = false;
     return AWCPML_OK;
 }
 
@@ -773,6 +791,7 @@ int awcpml_set_receivers(awcpml_plan* P, int shot, int nrecs, const int64_t* pos
     for (int r = 0; r < nrecs; ++r) P->rec_lin[shot][r] = lin_index(P, pos, nrecs, r);
     P->rec_set[shot] = true;
     P->io_dirty = true;
+    P->ck_valid = false;
     return AWCPML_OK;
 }
 
@@ -1118,6 +1137,341 @@ int awcpml_get_tiling(const awcpml_plan* P, int* block_y, int* march_len, int* p
     *march_len = P->march_len;
     *prefetch = P->prefetch;
     return AWCPML_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------
+// Adjoint (FWI gradient): checkpointed forward, then per checkpoint
+// segment a forward replay that stores q^s = d(pnew)/d(fact) for every
+// step of the segment, followed by the reverse-time steps of the segment
+// (adjoint_kernels.h).  Memory: ck_count checkpoints of 3 fields + slabs,
+// ck_every q fields, 3 adjoint fields, 2 x slabs, one gradient per shot.
+// ---------------------------------------------------------------------
+namespace {
+
+struct CkLayout {
+    size_t field;            // bytes of one field buffer (all shots)
+    size_t psi[3], xi[3];    // bytes of one side's slab (all shots)
+    size_t total;
+};
+
+CkLayout ck_layout(const awcpml_plan* P) {
+    CkLayout L{};
+    L.field = (size_t)P->field_elems * P->nshots * P->esize;
+    L.total = 3 * L.field;
+    for (int a = 0; a < P->ndim; ++a) {
+        L.psi[a] = (size_t)P->psi_elems[a] * P->nshots * P->esize;
+        L.xi[a] = (size_t)P->xi_elems[a] * P->nshots * P->esize;
+        L.total += 2 * (L.psi[a] + L.xi[a]);
+    }
+    return L;
+}
+
+// copy the plan state to (save) or from (restore) checkpoint c
+int ck_copy(awcpml_plan* P, int64_t c, bool save) {
+    const CkLayout L = ck_layout(P);
+    char* base = (char*)P->ck + (size_t)c * L.total;
+    auto mv = [&](void* state, size_t bytes) -> int {
+        if (bytes == 0) return AWCPML_OK;
+        void* dst = save ? (void*)base : state;
+        const void* src = save ? (const void*)base : (const void*)state;
+        HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, P->stream));
+        base += bytes;
+        return AWCPML_OK;
+    };
+    int st;
+    const int par = save ? P->par : P->ck_par[c];
+    for (int b = 0; b < 3; ++b)
+        if ((st = mv(P->p[b], L.field))) return st;
+    for (int a = 0; a < P->ndim; ++a)
+        for (int s = 0; s < 2; ++s) {
+            if ((st = mv(P->psi[a][s][par], L.psi[a]))) return st;
+            if ((st = mv(P->xi[a][s], L.xi[a]))) return st;
+        }
+    if (save) {
+        P->ck_rot[c] = P->rot;
+        P->ck_par[c] = P->par;
+    } else {
+        P->rot = P->ck_rot[c];
+        P->par = P->ck_par[c];
+        P->it = c * P->ck_every;
+    }
+    return AWCPML_OK;
+}
+
+int check_adjoint_shape(const awcpml_plan* P) {
+    for (int a = 0; a < P->ndim; ++a)
+        if ((P->active[a][0] || P->active[a][1]) && P->n[a] < 2 * (int64_t)P->halo + 4)
+            return fail(AWCPML_ERR_ARG,
+                        "adjoint needs >= 2*halo+4 = %d points on absorbing axis %d (has %lld)",
+                        2 * P->halo + 4, a, (long long)P->n[a]);
+    return AWCPML_OK;
+}
+
+// device buffers of one adjoint call
+struct AdjBuffers {
+    void* lam[3] = {};
+    void* al[3][2][2] = {};
+    void* be[3][2][2] = {};
+    void* q = nullptr;
+    void* grad = nullptr;
+    void* tbar = nullptr;
+    void* node_lin = nullptr;
+    void* node_start = nullptr;
+    void* rec_start = nullptr;
+    void* rec_idx = nullptr;
+    ~AdjBuffers() {
+        for (auto& v : lam) dfree(v);
+        for (int a = 0; a < 3; ++a)
+            for (int s = 0; s < 2; ++s)
+                for (int b = 0; b < 2; ++b) {
+                    dfree(al[a][s][b]);
+                    dfree(be[a][s][b]);
+                }
+        dfree(q);
+        dfree(grad);
+        dfree(tbar);
+        dfree(node_lin);
+        dfree(node_start);
+        dfree(rec_start);
+        dfree(rec_idx);
+    }
+};
+
+template <typename T>
+awcpml::AdjointArgs<T> make_adj_args(const awcpml_plan* P, const AdjBuffers& Bf, int lr, int par,
+                                     const void* q) {
+    awcpml::AdjointArgs<T> A;
+    memset(&A, 0, sizeof A);
+    for (int a = 0; a < 3; ++a) A.n[a] = P->n[a];
+    A.px = P->px;
+    A.halo = P->halo;
+    A.field_stride = P->field_elems;
+    A.lam0 = (T*)Bf.lam[lr];
+    A.lam1 = (const T*)Bf.lam[(lr + 1) % 3];
+    A.lam2 = (const T*)Bf.lam[(lr + 2) % 3];
+    A.fact = (const T*)P->fact;
+    A.q = (const T*)q;
+    A.grad = (T*)Bf.grad;
+    for (int a = 0; a < 3; ++a) {
+        A.inv_d[a] = (T)P->inv_d[a];
+        A.inv_d2[a] = (T)P->inv_d2[a];
+        A.psi_stride[a] = P->psi_elems[a];
+        A.xi_stride[a] = P->xi_elems[a];
+        for (int s = 0; s < 2; ++s) {
+            A.active[a][s] = (a < P->ndim) ? P->active[a][s] : 0;
+            if (a >= P->ndim) continue;
+            awcpml::AdjSide<T>& S = A.side[a][s];
+            S.al_rd = (const T*)Bf.al[a][s][par];
+            S.al_wr = (T*)Bf.al[a][s][par ^ 1];
+            S.be_rd = (const T*)Bf.be[a][s][par];
+            S.be_wr = (T*)Bf.be[a][s][par ^ 1];
+            S.a = (const T*)P->coef[a][0 + s];
+            S.bK = (const T*)P->coef[a][2 + s];
+            S.ah = (const T*)P->coef[a][4 + s];
+            S.bKh = (const T*)P->coef[a][6 + s];
+        }
+    }
+    A.ndim = P->ndim;
+    A.nshots = P->nshots;
+    A.march_len = 16;
+    const int64_t mext = P->ndim == 3 ? P->n[2] : P->n[1];
+    A.nchunks = (int)((mext - 2 + A.march_len - 1) / A.march_len);
+    const int64_t rows = P->ndim == 3 ? P->n[1] - 2 : 1;
+    A.total_threads = (int64_t)P->nshots * A.nchunks * rows * (P->n[0] - 2);
+    return A;
+}
+
+template <typename T>
+int adjoint_t(awcpml_plan* P, const double* tbar, double* fbar) {
+    const int64_t nt = P->nt, K = P->ck_every;
+    const size_t fb = (size_t)P->field_elems * P->nshots * sizeof(T);
+    AdjBuffers Bf;
+    int st;
+    for (int b = 0; b < 3; ++b) {
+        if ((st = dalloc(&Bf.lam[b], fb))) return st;
+        HIP_TRY(hipMemsetAsync(Bf.lam[b], 0, fb, P->stream));
+    }
+    for (int a = 0; a < P->ndim; ++a)
+        for (int s = 0; s < 2; ++s)
+            for (int b = 0; b < 2; ++b) {
+                const size_t pb = (size_t)P->psi_elems[a] * P->nshots * sizeof(T);
+                const size_t xb = (size_t)P->xi_elems[a] * P->nshots * sizeof(T);
+                if ((st = dalloc(&Bf.al[a][s][b], pb))) return st;
+                if ((st = dalloc(&Bf.be[a][s][b], xb))) return st;
+                HIP_TRY(hipMemsetAsync(Bf.al[a][s][b], 0, pb, P->stream));
+                if (xb) HIP_TRY(hipMemsetAsync(Bf.be[a][s][b], 0, xb, P->stream));
+            }
+    if ((st = dalloc(&Bf.q, fb * (size_t)K))) return st;
+    if ((st = dalloc(&Bf.grad, fb))) return st;
+    HIP_TRY(hipMemsetAsync(Bf.grad, 0, fb, P->stream));
+
+    // trace cotangents and the distinct interior receiver nodes per shot
+    const size_t ntr = (size_t)P->n_rec_total * nt;
+    if ((st = dalloc(&Bf.tbar, ntr * sizeof(T)))) return st;
+    if ((st = upload(P, Bf.tbar, tbar, ntr))) return st;
+    std::vector<int64_t> node_lin;
+    std::vector<int> node_start(1, 0), rec_start(1, 0), rec_idx;
+    int gr = 0;
+    for (int s = 0; s < P->nshots; ++s) {
+        const auto& R = P->rec_lin[s];
+        std::vector<std::pair<int64_t, int>> v;
+        for (size_t r = 0; r < R.size(); ++r) {
+            const int64_t lin = R[r];
+            const int64_t i = lin % P->px, j = (lin / P->px) % P->n[1], k = lin / (P->px * P->n[1]);
+            bool inside = i >= 1 && i <= P->n[0] - 2 && j >= 1 && j <= P->n[1] - 2;
+            if (P->ndim == 3) inside = inside && k >= 1 && k <= P->n[2] - 2;
+            if (inside) v.push_back({lin, gr + (int)r});
+        }
+        gr += (int)R.size();
+        std::stable_sort(v.begin(), v.end(),
+                         [](const std::pair<int64_t, int>& x, const std::pair<int64_t, int>& y) {
+                             return x.first < y.first;
+                         });
+        for (size_t e = 0; e < v.size(); ++e) {
+            if (e == 0 || v[e].first != v[e - 1].first) {
+                if (e) rec_start.push_back((int)rec_idx.size());
+                node_lin.push_back(v[e].first);
+            }
+            rec_idx.push_back(v[e].second);
+        }
+        if (!v.empty()) rec_start.push_back((int)rec_idx.size());
+        node_start.push_back((int)node_lin.size());
+    }
+    if ((st = dalloc(&Bf.node_lin, node_lin.size() * 8))) return st;
+    if ((st = dalloc(&Bf.node_start, node_start.size() * 4))) return st;
+    if ((st = dalloc(&Bf.rec_start, rec_start.size() * 4))) return st;
+    if ((st = dalloc(&Bf.rec_idx, rec_idx.size() * 4))) return st;
+    if (!node_lin.empty()) {
+        HIP_TRY(hipMemcpyAsync(Bf.node_lin, node_lin.data(), node_lin.size() * 8, hipMemcpyHostToDevice, P->stream));
+        HIP_TRY(hipMemcpyAsync(Bf.rec_idx, rec_idx.data(), rec_idx.size() * 4, hipMemcpyHostToDevice, P->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(Bf.node_start, node_start.data(), node_start.size() * 4, hipMemcpyHostToDevice, P->stream));
+    HIP_TRY(hipMemcpyAsync(Bf.rec_start, rec_start.data(), rec_start.size() * 4, hipMemcpyHostToDevice, P->stream));
+
+    awcpml::AdjIoArgs<T> io;
+    memset(&io, 0, sizeof io);
+    io.fact = (const T*)P->fact;
+    io.field_stride = P->field_elems;
+    io.node_lin = (const int64_t*)Bf.node_lin;
+    io.node_start = (const int*)Bf.node_start;
+    io.rec_start = (const int*)Bf.rec_start;
+    io.rec_idx = (const int*)Bf.rec_idx;
+    io.tbar = (const T*)Bf.tbar;
+    io.nt = nt;
+    for (int a = 0; a < 3; ++a) io.n[a] = P->n[a];
+    io.px = P->px;
+    io.halo = P->halo;
+    io.ndim = P->ndim;
+    io.grad = (T*)Bf.grad;
+
+    int lr = 0, par = 0;
+    for (int64_t seg = P->ck_count - 1; seg >= 0; --seg) {
+        const int64_t c0 = seg * K, c1 = std::min<int64_t>(c0 + K, nt);
+        if ((st = ck_copy(P, seg, false))) return st;
+        for (int64_t s = c0 + 1; s <= c1; ++s) {  // replay, storing q^s
+            P->q_cur = (char*)Bf.q + (size_t)(s - c0 - 1) * fb;
+            st = do_step(P, s - 1);
+            P->q_cur = nullptr;
+            if (st) return st;
+            ++P->it;
+        }
+        for (int64_t s = c1; s > c0; --s) {
+            const void* q = (const char*)Bf.q + (size_t)(s - c0 - 1) * fb;
+            awcpml::AdjointArgs<T> A = make_adj_args<T>(P, Bf, lr, par, q);
+            const int64_t nb = (A.total_threads + 255) / 256;
+            if (P->ndim == 3)
+                awcpml::adjoint_step<T, 3><<<dim3((unsigned)nb), dim3(256), 0, P->stream>>>(A);
+            else
+                awcpml::adjoint_step<T, 2><<<dim3((unsigned)nb), dim3(256), 0, P->stream>>>(A);
+            HIP_TRY(hipGetLastError());
+            if (!node_lin.empty()) {
+                io.lam0 = A.lam0;
+                io.q = (const T*)q;
+                for (int a = 0; a < 3; ++a)
+                    for (int sd = 0; sd < 2; ++sd) {
+                        io.side[a][sd] = A.side[a][sd];
+                        io.active[a][sd] = A.active[a][sd];
+                        io.xi_stride[a] = A.xi_stride[a];
+                    }
+                awcpml::adjoint_inject<T><<<dim3(P->nshots), dim3(256), 0, P->stream>>>(io, s - 1);
+                HIP_TRY(hipGetLastError());
+            }
+            lr = (lr + 2) % 3;
+            par ^= 1;
+        }
+    }
+    if (P->nshots > 1) {
+        const int64_t nb = (P->field_elems + 255) / 256;
+        awcpml::sum_shots<T><<<dim3((unsigned)nb), dim3(256), 0, P->stream>>>((T*)Bf.grad, P->field_elems,
+                                                                            P->nshots);
+        HIP_TRY(hipGetLastError());
+    }
+    std::vector<T> host((size_t)P->field_elems);
+    HIP_TRY(hipMemcpyAsync(host.data(), Bf.grad, host.size() * sizeof(T), hipMemcpyDeviceToHost, P->stream));
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    const int64_t nx = P->n[0], rows = P->n[1] * P->n[2];
+    for (int64_t r = 0; r < rows; ++r)
+        for (int64_t i = 0; i < nx; ++i) fbar[r * nx + i] = (double)host[(size_t)(r * P->px + i)];
+    return AWCPML_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int awcpml_checkpoint_run(awcpml_plan* P, int64_t every) {
+    g_err.clear();
+    int st = check_plan(P);
+    if (st) return st;
+    if ((st = ready(P))) return st;
+    if ((st = check_adjoint_shape(P))) return st;
+    if (P->nt < 1) return fail(AWCPML_ERR_STATE, "sources not set (nt unknown)");
+    if (every <= 0) every = std::max<int64_t>(1, (int64_t)llround(sqrt((double)P->nt)));
+    every = std::min<int64_t>(every, P->nt);
+    const int64_t count = (P->nt + every - 1) / every;
+    const CkLayout L = ck_layout(P);
+    const size_t need = L.total * (size_t)count;
+    if (need > P->ck_alloc) {
+        dfree(P->ck);
+        P->ck_alloc = 0;
+        if ((st = dalloc(&P->ck, need))) return st;
+        P->ck_alloc = need;
+    }
+    P->ck_bytes = L.total;
+    P->ck_every = every;
+    P->ck_count = count;
+    P->ck_rot.assign(count, 0);
+    P->ck_par.assign(count, 0);
+    P->ck_valid = false;
+    if ((st = awcpml_reset(P))) return st;
+    for (int64_t c = 0; c < count; ++c) {
+        if ((st = ck_copy(P, c, true))) return st;
+        const int64_t n = std::min<int64_t>(every, P->nt - P->it);
+        for (int64_t q = 0; q < n; ++q) {
+            if ((st = do_step(P, P->it))) return st;
+            ++P->it;
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    P->ck_valid = true;
+    return AWCPML_OK;
+}
+
+int awcpml_adjoint(awcpml_plan* P, const double* trace_bar, int64_t count, double* fact_bar) {
+    g_err.clear();
+    int st = check_plan(P);
+    if (st) return st;
+    if (!P->ck_valid) return fail(AWCPML_ERR_STATE, "no valid checkpoints: call awcpml_checkpoint_run first");
+    if (!fact_bar || (!trace_bar && count)) return fail(AWCPML_ERR_ARG, "NULL argument");
+    if (count != P->n_rec_total * P->nt)
+        return fail(AWCPML_ERR_ARG, "trace cotangent has %lld values, expected %lld (receivers x nt)",
+                    (long long)count, (long long)(P->n_rec_total * P->nt));
+    st = P->dtype == AWCPML_F64 ? adjoint_t<double>(P, trace_bar, fact_bar)
+                                : adjoint_t<float>(P, trace_bar, fact_bar);
+    P->q_cur = nullptr;
+    return st;
 }
 
 }  // extern "C"

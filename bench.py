@@ -93,7 +93,7 @@ def time_variant(S, dtype, nshots, steps, warmup, freetop, kernel_launches=50, t
         "shot_gpts_per_s": pts / t / 1e9,
         "effective_gbs": B * S.nt / t / 1e9,
         "kernel_ms": ms, "kernel_gbs": B / (ms * 1e-3) / 1e9,
-        "bytes_per_launch": B, "tiling": {"block_y": tiling[0], "march_len": tiling[1]},
+        "bytes_per_launch": B, "tiling": {"block_y": tiling[0], "march_len": tiling[1], "prefetch": tiling[2]},
         "_traces0": traces0,
     }
 
@@ -138,6 +138,31 @@ def load_traffic(kernel, workload):
     return None
 
 
+def profile_kernel(S, dtype, nshots, tiling, launches):
+    """Profiler target: only the stencil launches of one variant at a fixed
+    launch shape (no autotuning, so every dispatch of the kernel is the same
+    shape), then a 1 GiB FP32 device copy (1 GiB read + 1 GiB written) that
+    calibrates the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    P = build_plan(S, dtype, nshots)
+    if tiling:
+        P.set_tiling(*tiling)
+    P.reset()
+    ms = P.time_stencil(launches) / launches
+    esize = 8 if dtype == "f64" else 4
+    B, _ = algorithmic_bytes(S.n, S.halo, True, esize, nshots)
+    print(f"[profile] {dtype} S={nshots} tiling={P.tiling()} {launches} launches "
+          f"{ms:.4f} ms/launch, {B} algorithmic bytes/launch, {B / ms / 1e6:.1f} GB/s", flush=True)
+    P.close()
+    n = 1 << 28
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"[profile] calibration copy: 3 x {n * 4} bytes read + written", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +172,11 @@ def main():
     ap.add_argument("--nt", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--profile-kernel", action="store_true",
+                    help="profiler target: stencil launches of the FP32 S-shot variant only")
+    ap.add_argument("--profile-dtype", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--tiling", default="", help="block_y,march_len,prefetch for --profile-kernel")
+    ap.add_argument("--launches", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -164,6 +194,11 @@ def main():
     S = setup(case["l"], case["lt"], case["vel"], srcs, recs, halo=case["halo"],
               rcoef=case["rcoef"], ppw=case["ppw"], freetop=case["freetop"])
     print(f"[bench] plain3d n={S.n} nt={S.nt} halo={S.halo}", file=sys.stderr, flush=True)
+    if args.profile_kernel:
+        tiling = tuple(int(v) for v in args.tiling.split(",")) if args.tiling else None
+        nsh = args.shots if args.profile_dtype == "f32" else 1
+        profile_kernel(S, args.profile_dtype, nsh, tiling, args.launches)
+        return
 
     v64 = time_variant(S, "f64", 1, args.steps, args.warmup, case["freetop"])
     print(f"[bench] fp64 plain: {v64['shot_gpts_per_s']:.2f} shot*Gpts/s", file=sys.stderr, flush=True)
@@ -202,6 +237,7 @@ def main():
                            for k, v in v64.items() if not k.startswith("_")},
             "fp32_single": {k: (round(v, 4) if isinstance(v, float) else v)
                             for k, v in v32.items() if not k.startswith("_")},
+            "tiling": main_v["tiling"],
             "fp32_vs_fp64_plain": round(main_v["shot_gpts_per_s"] / v64["shot_gpts_per_s"], 3),
             "fp32_trace_rel_l2_vs_fp64": fp32_err,
         },

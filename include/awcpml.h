@@ -181,6 +181,34 @@ int awcpml_batch_time_advance(awcpml_batch *batch, int64_t nsteps, double *ms_to
  * advance without sources; call awcpml_batch_reset afterwards. */
 int awcpml_batch_time_stencil(awcpml_batch *batch, int nlaunches, double *ms_total);
 
+/* ------------------------------------------------------------------ */
+/* Differentiation: the FWI gradient by a hand-written adjoint.        */
+/* ------------------------------------------------------------------ */
+
+/* Reset the plan and run all nt steps, keeping a checkpoint of the forward
+ * state (the three field buffers, psi, xi) every `every` steps (<= 0:
+ * round(sqrt(nt))).  The traces of the run are then available as after
+ * awcpml_advance.  Device memory: ceil(nt/every) x (3 fields + slabs).
+ * Needs >= 2*halo+4 points on every absorbing axis.  The checkpoints stay
+ * valid until the model, the CPML coefficients, the sources or the
+ * receivers change.  This is the forward half of a reverse-mode rule: the
+ * reference has no gradient yet (README.md:297 lists the adjoint and
+ * checkpointing as work in progress); the Julia side would expose the pair
+ * as a ChainRulesCore rrule of the solve. */
+int awcpml_checkpoint_run(awcpml_plan *plan, int64_t every);
+
+/* Pullback of the traces: given trace_bar = dJ/dtraces (for every shot in
+ * order, that shot's receivers x nt values, time fastest -- the layout of
+ * awcpml_get_traces; count = total receivers x nt), write fact_bar =
+ * dJ/dfact summed over the shots (nx*ny*nz values, x fastest).  dt, the
+ * CPML coefficients and the source wavelets are held fixed; for the
+ * velocity, dJ/dvel = fact_bar * 2 * dt^2 * vel.  Per checkpoint segment
+ * the forward is replayed (storing d(pnew)/d(fact) for each step) and then
+ * stepped back in reverse time.  Receivers on Dirichlet nodes carry no
+ * gradient (those nodes do not depend on fact).  Leaves the plan's forward
+ * state at an intermediate step; the traces are unchanged. */
+int awcpml_adjoint(awcpml_plan *plan, const double *trace_bar, int64_t count, double *fact_bar);
+
 /* Message of the calling thread's last failed call ("" if none). */
 const char *awcpml_last_error(void);
 

@@ -1,0 +1,179 @@
+"""GPU adjoint (FWI gradient) against the oracle's FP64 CPU adjoint and
+against central finite differences of the GPU forward.
+
+FP64: the GPU gradient equals the oracle's to 1e-10 relative L2 (the two
+sum the same terms in different orders, so not bit for bit); the misfit is
+bitwise the oracle's because the FP64 forward is.  FP32: within 1e-3
+relative L2 of the FP64 oracle gradient (the measured gap is printed).
+"""
+import numpy as np
+import pytest
+
+import awcpml
+from awcpml import _lib, fwi
+from cases import oracle_problem, rel_l2, small2d, small3d
+from test_gpu_features import oracle_for, plan_from_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_grad(case, fact_scale=1.02, obs_scale=0.5, pos=None):
+    o = oracle_for(case)
+    o.run()
+    obs = o.traces() * obs_scale
+    f = o.fact() * fact_scale
+    o.set_fact(f)
+    J, g = o.fwi_gradient(obs)
+    return o, f, obs, J, g
+
+
+CASES = {
+    "2d_freetop": lambda: small2d(freetop=True, nt=160),
+    "2d_open": lambda: small2d(freetop=False, n=(41, 37), halo=5, nt=120),
+    "3d_freetop": lambda: small3d(freetop=True, nt=60),
+    "3d_open_2src": lambda: small3d(freetop=False, nt=50, nsrc=2),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_fp64_gradient_matches_oracle_adjoint(name):
+    case = CASES[name]()
+    print(f"[gpu] adjoint f64 {name}", flush=True)
+    o, f, obs, J, g = _oracle_grad(case)
+    P = plan_from_oracle(o)
+    sp = o._spacing
+    P.set_model(f, [1 / d for d in sp], [1 / (d * d) for d in sp])
+    for every in (0, 1, 7, o.nt):
+        Jg, gg = fwi.misfit_gradient(P, obs, every=every)
+        assert Jg == pytest.approx(J, rel=1e-13), every
+        err = rel_l2(gg, g)
+        print(f"  every={every} grad rel_l2 {err:.2e}", flush=True)
+        assert err <= 1e-10, (every, err)
+    P.close()
+
+
+@pytest.mark.parametrize("name", ["2d_freetop", "3d_freetop"])
+def test_fp32_gradient_close_to_fp64_oracle(name):
+    case = CASES[name]()
+    print(f"[gpu] adjoint f32 {name}", flush=True)
+    o, f, obs, J, g = _oracle_grad(case)
+    P = plan_from_oracle(o, dtype="f32")
+    sp = o._spacing
+    P.set_model(f, [1 / d for d in sp], [1 / (d * d) for d in sp])
+    Jg, gg = fwi.misfit_gradient(P, obs)
+    err = rel_l2(gg, g)
+    print(f"  f32 grad rel_l2 {err:.2e}  J rel {abs(Jg - J) / J:.2e}", flush=True)
+    assert abs(Jg - J) <= 1e-3 * J
+    assert err <= 1e-3
+    P.close()
+
+
+def test_multishot_gradient_is_sum_of_shots():
+    case = small3d(freetop=False, nt=45)
+    print("[gpu] adjoint multishot", flush=True)
+    o = oracle_for(case)
+    ps, _ = o.positions()
+    pos = []
+    for d in (0, 4, -3):
+        q = ps.copy()
+        q[:, 0] += d
+        pos.append(q)
+    M = plan_from_oracle(o, nshots=3, pos_list=pos)
+    M.checkpoint_run()
+    obs = [M.traces(s) * 0.3 for s in range(3)]
+    J, g = fwi.misfit_gradient(M, obs, every=5)
+    total = np.zeros(o.n)
+    Jsum = 0.0
+    for s in range(3):
+        S1 = plan_from_oracle(o, nshots=1, pos_list=[pos[s]])
+        Js, gs = fwi.misfit_gradient(S1, obs[s], every=9)
+        total += gs
+        Jsum += Js
+        S1.close()
+    assert J == pytest.approx(Jsum, rel=1e-13)
+    assert rel_l2(g, total) <= 1e-12
+    M.close()
+
+
+def test_gradient_matches_central_differences_on_gpu():
+    case = small2d(n=(49, 45), halo=6, nt=140, freetop=True)
+    print("[gpu] adjoint FD", flush=True)
+    o = oracle_for(case)
+    o.run()
+    obs = o.traces() * 0.5
+    P = plan_from_oracle(o)
+    sp = o._spacing
+    inv_d = [1 / d for d in sp]
+    inv_d2 = [1 / (d * d) for d in sp]
+    base = o.fact() * 1.02
+    P.set_model(base, inv_d, inv_d2)
+    J, g = fwi.misfit_gradient(P, obs, every=12)
+    rng = np.random.Generator(np.random.PCG64(21))
+    d = rng.standard_normal(o.n) * o.fact() * 0.01
+    eps = 1e-3
+    Js = []
+    for sgn in (1, -1):
+        P.set_model(base + sgn * eps * d, inv_d, inv_d2)
+        P.reset()
+        P.advance(o.nt)
+        r = P.traces() - obs
+        Js.append(0.5 * float(np.sum(r * r)))
+    fd = (Js[0] - Js[1]) / (2 * eps)
+    ad = float(np.sum(g * d))
+    print(f"  fd {fd:.12e} ad {ad:.12e}", flush=True)
+    assert abs(fd - ad) <= 1e-7 * abs(fd)
+    P.close()
+
+
+def test_torch_function_backward():
+    torch = pytest.importorskip("torch")
+    from awcpml.host import setup
+    case = small2d(nt=120)
+    print("[gpu] adjoint torch.autograd.Function", flush=True)
+    srcs = awcpml.Sources(case["srcpos"], case["t0s"], [awcpml.rickersource1D] * 2, case["f0"])
+    recs = awcpml.Receivers(case["recpos"])
+    S = setup(case["l"], case["lt"], case["vel"], srcs, recs, halo=case["halo"],
+              rcoef=case["rcoef"], ppw=case["ppw"], freetop=case["freetop"])
+    prob = fwi.Problem(S, every=10)
+    vel0 = np.asarray(case["vel"], dtype=np.float64)
+    obs = None
+    with torch.no_grad():
+        obs = fwi.solve_traces(torch.from_numpy(vel0.copy()), prob) * 0.5
+    v = torch.from_numpy(vel0 * 1.01).requires_grad_(True)
+    tr = fwi.solve_traces(v, prob)
+    loss = 0.5 * ((tr - obs) ** 2).sum()
+    loss.backward()
+    # same gradient through the numpy API
+    prob.set_velocity(vel0 * 1.01)
+    J, gf = fwi.misfit_gradient(prob.plan, obs[0].numpy())
+    gv = fwi.velocity_gradient(gf, vel0 * 1.01, S.dt)
+    assert float(loss) == pytest.approx(J, rel=1e-13)
+    assert rel_l2(v.grad.numpy(), gv) <= 1e-13
+    prob.close()
+
+
+def test_adjoint_errors():
+    case = small2d(nt=20)
+    print("[gpu] adjoint errors", flush=True)
+    o = oracle_for(case)
+    P = plan_from_oracle(o)
+    with pytest.raises(_lib.AwcpmlError, match="checkpoint"):
+        P.adjoint(np.zeros((o.nt, o.nrecs)))
+    P.checkpoint_run(4)
+    with pytest.raises(ValueError):
+        P.adjoint(np.zeros((o.nt + 1, o.nrecs)))
+    P.set_model(o.fact(), [1.0, 1.0], [1.0, 1.0])  # invalidates the checkpoints
+    with pytest.raises(_lib.AwcpmlError, match="checkpoint"):
+        P.adjoint(np.zeros((o.nt, o.nrecs)))
+    P.close()
+    # 15 points < 2*6+4 along an absorbing x axis
+    P2 = awcpml.Plan((15, 40), 6)
+    P2.set_model(np.full((15, 40), 0.1), [1.0, 1.0], [1.0, 1.0])
+    for a in range(2):
+        P2.set_cpml(a, [np.full(6, 0.1)] * 2 + [np.full(6, 0.5)] * 2 +
+                    [np.full(7, 0.1)] * 2 + [np.full(7, 0.5)] * 2)
+    P2.set_sources(0, [[7, 20]], np.zeros((10, 1)))
+    P2.set_receivers(0, [[7, 20]])
+    with pytest.raises(_lib.AwcpmlError, match="2\\*halo\\+4"):
+        P2.checkpoint_run()
+    P2.close()
