@@ -48,6 +48,8 @@ SIGNATURES = {
     "awcpml_get_tiling": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_int)]),
+    "awcpml_set_shot_group": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "awcpml_get_shot_group": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "awcpml_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
     "awcpml_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(_vp)]),
     "awcpml_batch_destroy": (ctypes.c_int, [_vp]),

@@ -1,6 +1,6 @@
 """Launch-shape autotuner for the stencil kernel.
 
-For a plan, time each candidate (block_y, march_len, prefetch) with HIP
+For a plan, time each candidate (block_y, march_len, prefetch, shot_group) with HIP
 events around a few stencil launches (``Plan.time_stencil``), keep the
 fastest, apply it to the plan and remember it in a JSON cache keyed by the
 problem's shape, halo, precision and shot count.  The wavefield state is
@@ -8,6 +8,9 @@ reset afterwards.
 
     from awcpml.autotune import autotune
     best = autotune(plan)            # {"block_y": 4, "march_len": 148, ...}
+
+Multi-shot plans also try shot groups (the waves of a block march the same
+rows of several shots, sharing the model rows they read).
 """
 import json
 import os
@@ -19,13 +22,17 @@ def candidates(plan):
     """Launch shapes worth timing for this plan's grid."""
     n_march = plan.n[-1] - 2 if plan.ndim == 3 else plan.n[1] - 2
     lens = sorted({min(n_march, v) for v in (8, 16, 32, 64, (n_march + 1) // 2, n_march)})
+    shapes = [(1, 1), (2, 1), (4, 1)]
+    if plan.nshots > 1:
+        shapes = [(by, g) for by in (1, 2, 4, 8) for g in (1, 2, 4, 8)
+                  if by % g == 0 and g <= plan.nshots]
     out = []
-    for by in (1, 2, 4):
+    for by, g in shapes:
         for ml in lens:
             for pf in (1, 2, 3):
                 if pf > 1 and ml < 2 * pf:
                     continue
-                out.append((by, ml, pf))
+                out.append((by, ml, pf, g))
     return out
 
 
@@ -49,19 +56,24 @@ def autotune(plan, cands=None, launches=8, cache=DEFAULT_CACHE, use_cache=True):
     if use_cache and key in db:
         best = db[key]
         plan.set_tiling(best["block_y"], best["march_len"], best["prefetch"])
+        plan.set_shot_group(best.get("shot_group", 1))
         return dict(best, cached=True)
     results = []
-    for by, ml, pf in (cands or candidates(plan)):
+    for cand in (cands or candidates(plan)):
+        by, ml, pf = cand[:3]
+        g = cand[3] if len(cand) > 3 else 1
         plan.set_tiling(by, ml, pf)
+        plan.set_shot_group(g)
         plan.reset()
         plan.time_stencil(2)  # warm-up
         ms = plan.time_stencil(launches) / launches
-        results.append((ms, by, ml, pf))
+        results.append((ms, by, ml, pf, g))
     plan.reset()
-    ms, by, ml, pf = min(results)
+    ms, by, ml, pf, g = min(results)
     plan.set_tiling(by, ml, pf)
-    best = {"block_y": by, "march_len": ml, "prefetch": pf, "ms_per_launch": ms,
-            "tried": len(results)}
+    plan.set_shot_group(g)
+    best = {"block_y": by, "march_len": ml, "prefetch": pf, "shot_group": g,
+            "ms_per_launch": ms, "tried": len(results)}
     if cache:
         db[key] = best
         try:

@@ -112,6 +112,14 @@ class Plan:
                                                  ctypes.byref(pf)))
         return by.value, ml.value, pf.value
 
+    def set_shot_group(self, shot_group):
+        _lib.check(_lib.load().awcpml_set_shot_group(self._h, int(shot_group)))
+
+    def shot_group(self):
+        g = ctypes.c_int()
+        _lib.check(_lib.load().awcpml_get_shot_group(self._h, ctypes.byref(g)))
+        return g.value
+
     # -- execution ----------------------------------------------------
     def reset(self):
         _lib.check(_lib.load().awcpml_reset(self._h))

@@ -110,6 +110,7 @@ struct awcpml_plan {
     std::vector<GraphEntry> graphs;
     int block_y = 0, march_len = 0;
     int prefetch = 2;  // software-pipeline depth of the march (1..3)
+    int shot_group = 1;  // shots per block sharing rows (divides block_y)
     int device = 0;
 
     // adjoint: forward-state checkpoints every ck_every steps (state after
@@ -217,18 +218,21 @@ StencilArgs<T> make_args(const awcpml_plan* P, int rot, int par) {
             S.bKh = (const T*)P->coef[a][6 + s];
         }
     }
-    const int by = P->block_y;
+    const int G = P->shot_group;
+    const int rows = P->block_y / G;  // rows (2D: chunks) per block
     A.march_len = P->march_len;
     A.nshots = P->nshots;
+    A.shot_group = G;
+    A.nsgroups = (P->nshots + G - 1) / G;
     A.tiles_x = (int)(round_up(P->n[0], awcpml::kTileX) / awcpml::kTileX);
     if (P->ndim == 3) {
-        A.tiles_y = (int)((P->n[1] - 2 + by - 1) / by);
+        A.tiles_y = (int)((P->n[1] - 2 + rows - 1) / rows);
         A.nchunks = (int)((P->n[2] - 2 + P->march_len - 1) / P->march_len);
-        A.logical_blocks = A.nshots * A.tiles_x * A.tiles_y * A.nchunks;
+        A.logical_blocks = A.nsgroups * A.tiles_x * A.tiles_y * A.nchunks;
     } else {
         A.nchunks = (int)((P->n[1] - 2 + P->march_len - 1) / P->march_len);
-        A.tiles_y = (A.nchunks + by - 1) / by;
-        A.logical_blocks = A.nshots * A.tiles_x * A.tiles_y;
+        A.tiles_y = (A.nchunks + rows - 1) / rows;
+        A.logical_blocks = A.nsgroups * A.tiles_x * A.tiles_y;
     }
     A.grid_blocks = (int)round_up(A.logical_blocks, awcpml::kNumXcd);
     return A;
@@ -977,15 +981,36 @@ int awcpml_set_tiling(awcpml_plan* P, int block_y, int march_len, int prefetch) 
     g_err.clear();
     int st = check_plan(P);
     if (st) return st;
-    if (block_y != 0 && block_y != 1 && block_y != 2 && block_y != 4)
-        return fail(AWCPML_ERR_ARG, "block_y must be 1, 2 or 4 (got %d)", block_y);
+    if (block_y != 0 && block_y != 1 && block_y != 2 && block_y != 4 && block_y != 8)
+        return fail(AWCPML_ERR_ARG, "block_y must be 1, 2, 4 or 8 (got %d)", block_y);
     if (march_len < 0) return fail(AWCPML_ERR_ARG, "march_len < 0");
     if (prefetch < 0 || prefetch > 3) return fail(AWCPML_ERR_ARG, "prefetch must be 0..3 (got %d)", prefetch);
     P->block_y = block_y;
     P->march_len = march_len;
     P->prefetch = prefetch ? prefetch : 2;
     drop_graphs(P->graphs);
-    return default_tiling(P);
+    int st2 = default_tiling(P);
+    if (P->block_y % P->shot_group) P->shot_group = 1;
+    return st2;
+}
+
+int awcpml_set_shot_group(awcpml_plan* P, int shot_group) {
+    g_err.clear();
+    int st = check_plan(P);
+    if (st) return st;
+    if (shot_group != 1 && shot_group != 2 && shot_group != 4 && shot_group != 8)
+        return fail(AWCPML_ERR_ARG, "shot_group must be 1, 2, 4 or 8 (got %d)", shot_group);
+    if (P->block_y % shot_group)
+        return fail(AWCPML_ERR_ARG, "shot_group %d does not divide block_y %d", shot_group, P->block_y);
+    P->shot_group = shot_group;
+    drop_graphs(P->graphs);
+    return AWCPML_OK;
+}
+
+int awcpml_get_shot_group(const awcpml_plan* P, int* shot_group) {
+    if (!P || !shot_group) return fail(AWCPML_ERR_ARG, "NULL argument");
+    *shot_group = P->shot_group;
+    return AWCPML_OK;
 }
 
 int awcpml_batch_create(awcpml_plan* const* plans, int count, awcpml_batch** out) {
@@ -1175,7 +1200,7 @@ int ck_copy(awcpml_plan* P, int64_t c, bool save) {
     auto mv = [&](void* state, size_t bytes) -> int {
         if (bytes == 0) return AWCPML_OK;
         void* dst = save ? (void*)base : state;
-        const void* src = save ? (const void*)base : (const void*)state;
+        const void* src = save ? (const void*)state : (const void*)base;
         HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, P->stream));
         base += bytes;
         return AWCPML_OK;

@@ -81,6 +81,10 @@ struct StencilArgs {
     int64_t psi_stride[3];    // elements between two shots' psi slabs
     int64_t xi_stride[3];
     int tiles_x, tiles_y, nchunks, march_len, nshots;
+    // shot_group waves of a block march the same rows of different shots
+    // (so the shared fact rows are fetched once per block); a block covers
+    // blockDim.y / shot_group rows (2D: march chunks) of nsgroups groups
+    int shot_group, nsgroups;
     int logical_blocks;
     int grid_blocks;          // launched blocks, a multiple of kNumXcd
     // optional: per node the derivative of pnew with respect to fact,
@@ -445,15 +449,19 @@ __device__ __forceinline__ void stencil3d_block(const StencilArgs<T>& A, int lb,
     __syncthreads();
 
     int rest = lb;
-    const int shot = rest % A.nshots; rest /= A.nshots;
+    const int sg = rest % A.nsgroups; rest /= A.nsgroups;
     const int tx = rest % A.tiles_x; rest /= A.tiles_x;
     const int ty = rest % A.tiles_y;
     const int chunk = rest / A.tiles_y;
     // blockDim.x == 64 == wavefront: threadIdx.y is the same on all lanes
     const int wy = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    const int G = A.shot_group;
+    const int shot = sg * G + wy % G;
+    const int rows = (int)blockDim.y / G;
     const int64_t i = (int64_t)tx * kTileX + threadIdx.x;
-    const int64_t j = 1 + (int64_t)ty * blockDim.y + wy;
+    const int64_t j = 1 + (int64_t)ty * rows + wy / G;
     const int64_t nx = A.nx, ny = A.ny, nz = A.nz, halo = A.halo;
+    if (shot >= A.nshots) return;
     if (j > ny - 2) return;
     if (i < 1 || i > nx - 2) return;
     const int kbeg = 1 + chunk * A.march_len;
@@ -511,13 +519,16 @@ __device__ __forceinline__ void stencil2d_block(const StencilArgs<T>& A, int lb,
     __syncthreads();
 
     int rest = lb;
-    const int shot = rest % A.nshots; rest /= A.nshots;
+    const int sg = rest % A.nsgroups; rest /= A.nsgroups;
     const int tx = rest % A.tiles_x; rest /= A.tiles_x;
-    const int ty = rest;  // group of blockDim.y chunks
+    const int ty = rest;  // group of blockDim.y / shot_group chunks
     const int wy = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
-    const int chunk = ty * blockDim.y + wy;
+    const int G = A.shot_group;
+    const int shot = sg * G + wy % G;
+    const int chunk = ty * ((int)blockDim.y / G) + wy / G;
     const int64_t i = (int64_t)tx * kTileX + threadIdx.x;
     const int64_t nx = A.nx, ny = A.ny, halo = A.halo;
+    if (shot >= A.nshots) return;
     if (chunk >= A.nchunks) return;
     if (i < 1 || i > nx - 2) return;
     const int jbeg = 1 + chunk * A.march_len;
@@ -584,7 +595,7 @@ __device__ __forceinline__ int batch_plan(const int* start, int nplans, int lb) 
 }
 
 template <typename T, typename Off, int D>
-__global__ __launch_bounds__(256) void stencil3d(const StencilArgs<T> A) {
+__global__ __launch_bounds__(512) void stencil3d(const StencilArgs<T> A) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int lb = logical_block(blockIdx.x, A.grid_blocks);
     if (lb >= A.logical_blocks) return;  // whole block
@@ -592,7 +603,7 @@ __global__ __launch_bounds__(256) void stencil3d(const StencilArgs<T> A) {
 }
 
 template <typename T, typename Off, int D>
-__global__ __launch_bounds__(256) void stencil2d(const StencilArgs<T> A) {
+__global__ __launch_bounds__(512) void stencil2d(const StencilArgs<T> A) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int lb = logical_block(blockIdx.x, A.grid_blocks);
     if (lb >= A.logical_blocks) return;  // whole block
@@ -600,7 +611,7 @@ __global__ __launch_bounds__(256) void stencil2d(const StencilArgs<T> A) {
 }
 
 template <typename T, typename Off, int D, int NDIM>
-__global__ __launch_bounds__(256) void stencil_batch(const BatchArgs<T> B) {
+__global__ __launch_bounds__(512) void stencil_batch(const BatchArgs<T> B) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int lb = logical_block(blockIdx.x, B.grid_blocks);
     if (lb >= B.block_start[B.nplans]) return;  // whole block

@@ -26,6 +26,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "acousticwavecpml.jl_amd"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -125,6 +127,151 @@ def cpu_baseline(case, max_seconds=20.0):
     }
 
 
+def _setup(case, vel=None, srcpos=None):
+    import awcpml
+    from awcpml.host import setup
+    vel = case["vel"] if vel is None else vel
+    srcpos = case["srcpos"] if srcpos is None else srcpos
+    srcs = awcpml.Sources(srcpos, case["t0s"] * len(srcpos), [awcpml.rickersource1D] * len(srcpos),
+                          case["f0"])
+    recs = awcpml.Receivers(case["recpos"])
+    return setup(case["l"], case["lt"], vel, srcs, recs, halo=case["halo"], rcoef=case["rcoef"],
+                 ppw=case["ppw"], freetop=case["freetop"])
+
+
+def _shot_plan(S, shots, dtype, first=0, count=None):
+    """Plan over shots[first:first+count] (one source each) on setup S."""
+    import awcpml
+    from awcpml.host import round_ties_up
+    count = len(shots) - first if count is None else count
+    P = awcpml.Plan(S.n, S.halo, dtype=dtype, nshots=count)
+    P.set_model(S.fact, S.inv_d, S.inv_d2)
+    for a in range(S.ndim):
+        P.set_cpml(a, S.coef[a])
+    for q in range(count):
+        pos = np.asarray(shots[first + q], dtype=np.float64)
+        idx = np.array([[int(round_ties_up(pos[0, a] / S.d[a] + 1)) for a in range(S.ndim)]])
+        P.set_sources(q, idx, S.dt2srctf)
+        P.set_receivers(q, S.posrecs)
+    return P
+
+
+def bench_batch2d(steps):
+    """Many shots on one 2D model in one launch per step, graph-captured
+    time loop; error of two shots against the FP64 path."""
+    import awcpml
+    from cases import batch2d, rel_l2
+    case = batch2d()
+    S = _setup(case)
+    P = _shot_plan(S, case["shots"], "f32")
+    tune = awcpml.autotune(P, launches=5)
+    P.set_graph(True)
+    P.reset()
+    P.advance(S.nt)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        P.reset()
+        P.advance(S.nt)
+    t = (time.perf_counter() - t0) / steps
+    P.reset()
+    ms = P.time_stencil(50) / 50
+    P.reset()
+    P.advance(S.nt)
+    e = 0.0
+    for q in (0, len(case["shots"]) - 1):
+        R = _shot_plan(S, case["shots"], "f64", first=q, count=1)
+        R.reset()
+        R.advance(S.nt)
+        e = max(e, rel_l2(P.traces(q), R.traces(0)))
+        R.close()
+    N = S.n[0] * S.n[1]
+    B, _ = algorithmic_bytes(S.n, S.halo, False, 4, P.nshots)
+    out = {"workload": f"2D {S.n[0]}^2 (1024^2 interior + 20 halo), {P.nshots} shots, nt={S.nt}, FP32",
+           "s_per_solve": round(t, 4), "shot_gpts_per_s": round(P.nshots * N * S.nt / t / 1e9, 2),
+           "kernel_ms": round(ms, 4), "kernel_gbs": round(B / ms / 1e6, 1),
+           "trace_rel_l2_vs_fp64_max": e, "tiling": [tune["block_y"], tune["march_len"], tune["prefetch"]]}
+    P.close()
+    return out
+
+
+def bench_uneven3d():
+    """12 3D models of uneven size in one launch per step (FP32), then each
+    model alone in FP64 for its trace error."""
+    import awcpml
+    from cases import lt_for_nt, rel_l2, uneven3d
+    cases = uneven3d()
+    plans, setups = [], []
+    for c in cases:
+        vel = c["make_vel"]()
+        c = dict(c)
+        c["lt"] = lt_for_nt(3, (10.0,) * 3, float(vel.max()), c["nt"])
+        S = _setup(c, vel=vel)
+        del vel
+        P = awcpml.Plan.from_setup(S, dtype="f32")
+        plans.append(P)
+        setups.append(S)
+    B = awcpml.Batch(plans)
+    B.set_graph(True)
+    B.run()
+    t0 = time.perf_counter()
+    B.run()
+    t = time.perf_counter() - t0
+    pts = sum(P.nt * int(np.prod(S.n)) for P, S in zip(plans, setups))
+    ms = B.time_stencil(10) / 10
+    bytes_launch = sum(algorithmic_bytes(S.n, S.halo, False, 4, 1)[0] for S in setups)
+    B.run()
+    errs = []
+    for P, S in zip(plans, setups):
+        R = awcpml.Plan.from_setup(S, dtype="f64")
+        R.reset()
+        R.advance(S.nt)
+        errs.append(rel_l2(P.traces(0), R.traces(0)))
+        R.close()
+    B.close()
+    for P in plans:
+        P.close()
+    return {"workload": f"{len(cases)} 3D models, edges {[S.n[0] for S in setups]} nodes, one shot "
+                        f"each, nt={setups[0].nt}, FP32, one batch launch per step",
+            "s_per_solve": round(t, 4), "shot_gpts_per_s": round(pts / t / 1e9, 2),
+            "kernel_ms": round(ms, 4), "kernel_gbs": round(bytes_launch / ms / 1e6, 1),
+            "trace_rel_l2_vs_fp64": [float("%.3g" % e) for e in errs]}
+
+
+def bench_fwi2d():
+    """FWI gradient of the L2 trace misfit (32 shots, 554^2, FP32) with
+    checkpointing every 50 steps, and its error against the FP64 gradient."""
+    from awcpml import fwi
+    from cases import fwi2d, rel_l2
+    case = fwi2d()
+    S = _setup(case)
+    obs_plan = _shot_plan(S, case["shots"], "f32")
+    obs_plan.reset()
+    obs_plan.advance(S.nt)
+    obs = [obs_plan.traces(q) for q in range(obs_plan.nshots)]
+    obs_plan.close()
+    fact0 = np.asfortranarray((S.dt * S.dt) * (case["start"] * case["start"]))
+    res = {}
+    for dtype in ("f32", "f64"):
+        P = _shot_plan(S, case["shots"], dtype)
+        P.set_model(fact0, S.inv_d, S.inv_d2)
+        fwi.misfit_gradient(P, obs, every=case["every"])  # warm-up
+        t0 = time.perf_counter()
+        J, g = fwi.misfit_gradient(P, obs, every=case["every"])
+        res[dtype] = (time.perf_counter() - t0, J, g)
+        P.close()
+    t32, J32, g32 = res["f32"]
+    N = S.n[0] * S.n[1]
+    gv = fwi.velocity_gradient(g32, case["start"], S.dt)
+    c = S.n[0] // 2
+    return {"workload": f"2D {S.n[0]}^2, {len(case['shots'])} shots, nt={S.nt}, FP32, checkpoint "
+                        f"every {case['every']} steps (forward, replay, adjoint)",
+            "s_per_gradient": round(t32, 4),
+            "shot_gpts_per_s_forward_equiv": round(len(case["shots"]) * N * S.nt / t32 / 1e9, 2),
+            "misfit": J32, "grad_rel_l2_vs_fp64": rel_l2(g32, res["f64"][2]),
+            "s_per_gradient_fp64": round(res["f64"][0], 4),
+            "vel_grad_sign_at_anomaly": float(np.sign(gv[c, c]))}
+
+
 def load_traffic(kernel, workload):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -171,6 +318,8 @@ def main():
     ap.add_argument("--shots", type=int, default=8)
     ap.add_argument("--nt", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched-2D, uneven-3D and FWI-gradient measurements")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--profile-kernel", action="store_true",
                     help="profiler target: stencil launches of the FP32 S-shot variant only")
@@ -249,6 +398,14 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if not args.no_extras:
+        extras = {}
+        for name, fn in (("batch2d", lambda: bench_batch2d(1)), ("uneven3d", bench_uneven3d),
+                         ("fwi2d", bench_fwi2d)):
+            print(f"[bench] extra: {name}", file=sys.stderr, flush=True)
+            extras[name] = fn()
+            print(f"[bench] {name}: {extras[name]}", file=sys.stderr, flush=True)
+        out["config"]["extras"] = extras
     if not args.no_cpu:
         print("[bench] cpu baseline (oracle)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)

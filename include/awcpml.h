@@ -137,14 +137,24 @@ int awcpml_time_stencil(awcpml_plan *plan, int nlaunches, double *ms_total);
  * plan's stream, in milliseconds.  Measurement support (see above). */
 int awcpml_time_advance(awcpml_plan *plan, int64_t nsteps, double *ms_total);
 
-/* Launch geometry of the stencil kernel: threads per block along y
- * (1, 2 or 4), the length of the z (3D) or y (2D) march per block, and the
- * software-pipeline depth of the march (1..3: loads issued up to prefetch-1
- * nodes ahead).  0 keeps the library's choice.  Used by the autotuner; no
- * reference counterpart (ParallelStencil picks its launch shape itself). */
+/* Launch geometry of the stencil kernel: wavefronts per block along y
+ * (1, 2, 4 or 8), the length of the z (3D) or y (2D) march per block, and
+ * the software-pipeline depth of the march (1..3: loads issued up to
+ * prefetch-1 nodes ahead).  0 keeps the library's choice.  A block_y that
+ * the current shot group does not divide resets the shot group to 1.  Used
+ * by the autotuner; no reference counterpart (ParallelStencil picks its
+ * launch shape itself). */
 int awcpml_set_tiling(awcpml_plan *plan, int block_y, int march_len, int prefetch);
 int awcpml_get_tiling(const awcpml_plan *plan, int *block_y, int *march_len,
                       int *prefetch);
+
+/* Shots per block (1, 2, 4 or 8, dividing block_y): the wavefronts of a
+ * block then march the same block_y/shot_group rows of shot_group
+ * different shots, so the model (fact) rows they share are fetched once
+ * per block.  Results do not depend on it.  Multi-shot extension, no
+ * reference counterpart. */
+int awcpml_set_shot_group(awcpml_plan *plan, int shot_group);
+int awcpml_get_shot_group(const awcpml_plan *plan, int *shot_group);
 
 /* Capture the time loop in HIP graphs: with enable != 0, awcpml_advance and
  * awcpml_time_advance record each (first step, number of steps) sequence

@@ -153,3 +153,81 @@ def rel_l2(a, b):
     b = np.asarray(b, dtype=np.float64)
     den = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / den) if den > 0 else float(np.linalg.norm(a - b))
+
+
+def batch2d(nt=3000, interior=1024, halo=20, dx=10.0, nshots=64, nrec=256, seed=3):
+    """Many shots on one 2D model (1066^2 nodes): layered gradient plus
+    seeded smoothed anomalies (+-8 %, sigma = 20 points), 64 Ricker shots
+    (f0 = 12 Hz) at 0-based depth 21 spread evenly along x, 256 receivers
+    at the same depth.  `shots` holds one source position per shot."""
+    n = interior + 2 * halo + 2
+    d = (dx, dx)
+    L = (n - 1) * dx
+    depth = np.arange(n) * dx
+    layer = np.minimum(3, (4 * np.arange(n)) // n)           # four layers by depth
+    base = (1800.0 + 500.0 * layer + 0.05 * depth)[None, :]
+    vel = np.asfortranarray(base * (1.0 + 0.08 * smooth_random((n, n), 20.0, seed)))
+    vmax = float(vel.max())
+    zs = (halo + 1) * dx
+    shots = [np.array([[i * dx, zs]]) for i in interior_grid(interior, halo, nshots)]
+    recs = np.array([[i * dx, zs] for i in interior_grid(interior, halo, nrec)])
+    return dict(n=[n, n], l=[L, L], lt=lt_for_nt(2, d, vmax, nt), vel=vel, srcpos=shots[0],
+                shots=shots, t0s=[1.2 / 12.0], f0=12.0, recpos=recs, halo=halo, rcoef=1e-3,
+                freetop=False, ppw=10.0, nt=nt)
+
+
+def uneven3d(nt=1500, halo=20, dx=10.0, count=12, seed=11, edges=(128, 192, 256, 384, 512)):
+    """`count` independent 3D models whose interior cube edges are drawn
+    uniformly (seeded) from `edges`; velocity 1500 m/s at the top rising to
+    4000 m/s at the bottom plus a +-500 m/s smooth analytic anomaly pattern
+    (range within 1500-4500 m/s); one Ricker shot (f0 = 10 Hz) each, 16
+    receivers on a line.  Returns a list of case dicts (built lazily: each
+    holds a `make_vel` callable instead of the array, to keep host memory
+    at one model at a time)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for m in range(count):
+        e = int(rng.choice(edges))
+        n = e + 2 * halo + 2
+        d = (dx, dx, dx)
+        L = (n - 1) * dx
+        phase = float(rng.uniform(0, 2 * np.pi))
+
+        def make_vel(n=n, phase=phase):
+            x = np.linspace(0.0, 1.0, n)
+            base = 1500.0 + 2500.0 * x
+            an = 500.0 * (np.sin(3 * np.pi * x + phase)[:, None, None] *
+                          np.sin(2 * np.pi * x)[None, :, None] *
+                          np.cos(3 * np.pi * x + 0.5 * phase)[None, None, :])
+            v = base[None, :, None] + an
+            return np.asfortranarray(np.clip(v, 1500.0, 4500.0))
+
+        c = (halo + 1 + e // 2) * dx
+        rec = [[(halo + 1 + (k + 0.5) * e / 16) * dx, (halo + 1 + 5) * dx, c] for k in range(16)]
+        # lt follows from the built model: lt_for_nt(3, (dx,)*3, vel.max(), nt)
+        out.append(dict(n=[n, n, n], l=[L, L, L], lt=None, make_vel=make_vel, srcpos=np.array([[c, (halo + 1 + 10) * dx, c]]), t0s=[0.12],
+                        f0=10.0, recpos=np.array(rec), halo=halo, rcoef=1e-3, freetop=False,
+                        ppw=10.0, nt=nt))
+    return out
+
+
+def fwi2d(nt=2500, interior=512, halo=20, dx=10.0, nshots=32, nrec=256, every=50):
+    """2D FWI gradient case (554^2 nodes): true model 2000 m/s with a +10 %
+    Gaussian anomaly (sigma = 30 points) at the centre, starting model
+    2000 m/s; 32 Ricker shots (f0 = 8 Hz) and 256 receivers at 0-based
+    depth 21; forward checkpoints every 50 steps."""
+    n = interior + 2 * halo + 2
+    d = (dx, dx)
+    L = (n - 1) * dx
+    c = (n - 1) / 2
+    ii, jj = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    r2 = (ii - c) ** 2 + (jj - c) ** 2
+    true = np.asfortranarray(2000.0 * (1.0 + 0.1 * np.exp(-r2 / (2 * 30.0 ** 2))))
+    start = np.full((n, n), 2000.0, order="F")
+    zs = (halo + 1) * dx
+    shots = [np.array([[i * dx, zs]]) for i in interior_grid(interior, halo, nshots)]
+    recs = np.array([[i * dx, zs] for i in interior_grid(interior, halo, nrec)])
+    # dt from the true model's vmax (held fixed while the model changes)
+    return dict(n=[n, n], l=[L, L], lt=lt_for_nt(2, d, float(true.max()), nt), vel=true,
+                start=start, srcpos=shots[0], shots=shots, t0s=[1.2 / 8.0], f0=8.0, recpos=recs,
+                halo=halo, rcoef=1e-3, freetop=False, ppw=10.0, nt=nt, every=every)

@@ -186,3 +186,41 @@ def test_forward_and_errors():
     P.forward(1)  # one forward! at it = 1, like the reference's do_bench
     assert P.steps_done() == o.nt
     P.close()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_shot_groups_are_exact(dim):
+    """Every (block_y, shot_group) shape gives the bits of the single-shot
+    plans, including shot counts that the group does not divide."""
+    case = small3d(n=(41, 37, 35), halo=6, nt=40, freetop=False) if dim == 3 else \
+        small2d(n=(61, 47), halo=7, nt=60, freetop=True)
+    print(f"[gpu] shot groups {dim}D", flush=True)
+    o = oracle_for(case)
+    ps, _ = o.positions()
+    pos = []
+    for d in (0, 3, -4, 5, -2):
+        q = ps.copy()
+        q[:, 0] += d
+        pos.append(q)
+    refs = []
+    for q in pos:
+        S = plan_from_oracle(o, nshots=1, pos_list=[q])
+        S.reset()
+        S.advance(o.nt)
+        refs.append((S.traces(), S.field()))
+        S.close()
+    M = plan_from_oracle(o, nshots=len(pos), pos_list=pos)
+    for by, g in ((1, 1), (2, 2), (4, 2), (4, 4), (8, 1), (8, 4), (8, 8)):
+        for ml, pf in ((5, 2), (16, 3)):
+            M.set_tiling(by, ml, pf)
+            M.set_shot_group(g)
+            assert M.shot_group() == g
+            M.reset()
+            M.advance(o.nt)
+            for s, (tr, fld) in enumerate(refs):
+                assert np.array_equal(M.traces(s), tr), (by, g, ml, pf, s)
+                assert np.array_equal(M.field(s), fld), (by, g, ml, pf, s)
+    with pytest.raises(_lib.AwcpmlError, match="does not divide"):
+        M.set_tiling(2, 8, 2)
+        M.set_shot_group(4)
+    M.close()
