@@ -29,8 +29,8 @@ def candidates(plan):
     out = []
     for by, g in shapes:
         for ml in lens:
-            for pf in (1, 2, 3):
-                if pf > 1 and ml < 2 * pf:
+            for pf in (1, 2, 3, 4, 5):
+                if (1 < pf <= 3 and ml < 2 * pf) or (pf > 3 and ml < 8):
                     continue
                 out.append((by, ml, pf, g))
     return out

@@ -255,6 +255,8 @@ void launch_depth(const awcpml_plan* P, const StencilArgs<T>& A) {
     switch (P->prefetch) {
         case 1: launch_variant<T, Off, 1>(P, A); break;
         case 3: launch_variant<T, Off, 3>(P, A); break;
+        case 4: launch_variant<T, Off, 4>(P, A); break;
+        case 5: launch_variant<T, Off, 5>(P, A); break;
         default: launch_variant<T, Off, 2>(P, A); break;
     }
 }
@@ -535,12 +537,16 @@ int batch_step_t(awcpml_batch* B, int64_t it0, bool with_io) {
         switch (B->prefetch) {
             case 1: batch_launch_variant<T, uint32_t, 1>(B, A); break;
             case 3: batch_launch_variant<T, uint32_t, 3>(B, A); break;
+            case 4: batch_launch_variant<T, uint32_t, 4>(B, A); break;
+            case 5: batch_launch_variant<T, uint32_t, 5>(B, A); break;
             default: batch_launch_variant<T, uint32_t, 2>(B, A); break;
         }
     } else {
         switch (B->prefetch) {
             case 1: batch_launch_variant<T, uint64_t, 1>(B, A); break;
             case 3: batch_launch_variant<T, uint64_t, 3>(B, A); break;
+            case 4: batch_launch_variant<T, uint64_t, 4>(B, A); break;
+            case 5: batch_launch_variant<T, uint64_t, 5>(B, A); break;
             default: batch_launch_variant<T, uint64_t, 2>(B, A); break;
         }
     }
@@ -984,7 +990,7 @@ int awcpml_set_tiling(awcpml_plan* P, int block_y, int march_len, int prefetch) 
     if (block_y != 0 && block_y != 1 && block_y != 2 && block_y != 4 && block_y != 8)
         return fail(AWCPML_ERR_ARG, "block_y must be 1, 2, 4 or 8 (got %d)", block_y);
     if (march_len < 0) return fail(AWCPML_ERR_ARG, "march_len < 0");
-    if (prefetch < 0 || prefetch > 3) return fail(AWCPML_ERR_ARG, "prefetch must be 0..3 (got %d)", prefetch);
+    if (prefetch < 0 || prefetch > 5) return fail(AWCPML_ERR_ARG, "prefetch must be 0..5 (got %d)", prefetch);
     P->block_y = block_y;
     P->march_len = march_len;
     P->prefetch = prefetch ? prefetch : 2;

@@ -108,6 +108,19 @@ template <typename T, typename Off>
 __device__ __forceinline__ void stg(T* base, Off byte_off, T v) {
     *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off) = v;
 }
+// Streaming variants for data touched once per launch (pold is read once,
+// pnew is written once and next read in the following launch): the
+// nontemporal hint keeps them from displacing the reused pcur and fact
+// lines in L2.
+template <typename T, typename Off>
+__device__ __forceinline__ T ldg_stream(const T* base, Off byte_off) {
+    return __builtin_nontemporal_load(
+        reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off));
+}
+template <typename T, typename Off>
+__device__ __forceinline__ void stg_stream(T* base, Off byte_off, T v) {
+    __builtin_nontemporal_store(v, reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off));
+}
 
 // Classify a node along one axis: side 0/1 and layer m, or side -1.
 // Mirrors `if i <= halo+1 ... elseif i >= nx-halo` (1-based) with the
@@ -281,7 +294,7 @@ __device__ __forceinline__ void load_node(const March<T, Off, NDIM>& M, int k, i
         q.ym = ldg(M.pcur, c - M.sy);
         q.yp = ldg(M.pcur, c + M.sy);
     }
-    q.po = ldg(M.pold, c);
+    q.po = ldg_stream(M.pold, c);
     q.f = ldg(M.fact, c);
     if (SA) {
         q.psa0 = ldg(M.sa.p.psi_rd, M.sa.o_psi + (Off)k * M.sa.st_psi);
@@ -346,7 +359,7 @@ __device__ __forceinline__ void compute_node(const March<T, Off, NDIM>& M, int k
     T lap;
     if (NDIM == 3) lap = d2a + d2b + d2m;   // (d2x + d2y) + d2z
     else lap = d2a + d2m;                   // d2x + d2y
-    stg(M.pnew, M.col + (Off)k * M.sm, two * pc - q.po + f * lap + damp);
+    stg_stream(M.pnew, M.col + (Off)k * M.sm, two * pc - q.po + f * lap + damp);
     if (M.qout) stg(M.qout, M.col + (Off)k * M.sm, lap + slab);
 }
 
@@ -399,6 +412,130 @@ __device__ __forceinline__ void march_segment(const March<T, Off, NDIM>& M, int 
     }
 }
 
+// Two-distance pipeline.  The loads of a node fall in two groups:
+//  - far: the new pcur plane ahead of the march, pold and fact, the only
+//    loads that normally come from HBM;
+//  - near: the in-plane neighbours (xm, xp, ym, yp: the same lines as pcur
+//    of this and the neighbouring rows, cached) and the slab values.
+// Far loads are issued F nodes ahead and near loads DN nodes ahead (DN <=
+// F), near before far within each node's issue group: a wave then waits
+// only for the near loads of the node it computes while the far loads of
+// the next F nodes stay in flight, which keeps more HBM bytes outstanding
+// per wave than one ring of whole-node slots at the same register cost.
+// pcur values along the march are kept in a register ring of CR >= F + 2
+// entries (each plane is loaded once and used as pp, pc and pm); pold/fact
+// use a ring of CR entries as well so every slot index is a compile-time
+// constant in the loop unrolled CR times (CR is a multiple of DN).
+template <int P>
+struct Pipe;
+template <>
+struct Pipe<4> {
+    static constexpr int CR = 5, F = 3, DN = 1;
+};
+template <>
+struct Pipe<5> {
+    static constexpr int CR = 6, F = 4, DN = 2;
+};
+
+template <typename T, typename Off, int NDIM, bool SA, bool SB, bool SM>
+__device__ __forceinline__ void load_near(const March<T, Off, NDIM>& M, int k, int mside,
+                                          NodeLoads<T>& q) {
+    constexpr Off E = (Off)sizeof(T);
+    const Off c = M.col + (Off)k * M.sm;
+    q.xm = ldg(M.pcur, c - E);
+    q.xp = ldg(M.pcur, c + E);
+    if (NDIM == 3) {
+        q.ym = ldg(M.pcur, c - M.sy);
+        q.yp = ldg(M.pcur, c + M.sy);
+    }
+    if (SA) {
+        q.psa0 = ldg(M.sa.p.psi_rd, M.sa.o_psi + (Off)k * M.sa.st_psi);
+        q.psa1 = ldg(M.sa.p.psi_rd, M.sa.o_psi1 + (Off)k * M.sa.st_psi);
+        q.xia = ldg((const T*)M.sa.p.xi, M.sa.o_xi + (Off)k * M.sa.st_xi);
+    }
+    if (SB) {
+        q.psb0 = ldg(M.sb.p.psi_rd, M.sb.o_psi + (Off)k * M.sb.st_psi);
+        q.psb1 = ldg(M.sb.p.psi_rd, M.sb.o_psi1 + (Off)k * M.sb.st_psi);
+        q.xib = ldg((const T*)M.sb.p.xi, M.sb.o_xi + (Off)k * M.sb.st_xi);
+    }
+    if (SM) {
+        const int mz = k - M.mshift[mside];
+        const SlabPtrs<T>& P = M.pm_side[mside];
+        q.psm0 = ldg(P.psi_rd, M.col + (Off)mz * M.sm);
+        q.psm1 = ldg(P.psi_rd, M.col + (Off)(mz - 1) * M.sm);
+        q.xim = ldg((const T*)P.xi, M.col + (Off)(mz - 1) * M.sm);
+    }
+}
+
+template <typename T, typename Off, int NDIM, int PK, bool SA, bool SB, bool SM>
+__device__ __forceinline__ void march_segment_far(const March<T, Off, NDIM>& M, int k_from, int k_to,
+                                                  int mside) {
+    constexpr int CR = Pipe<PK>::CR, F = Pipe<PK>::F, DN = Pipe<PK>::DN;
+    static_assert(CR >= F + 2 && CR % DN == 0 && DN <= F, "pipeline shape");
+    if (k_to - k_from < F + 2) {
+        for (int k = k_from; k < k_to; ++k) {
+            NodeLoads<T> q;
+            load_node<T, Off, NDIM, SA, SB, SM>(M, k, mside, q);
+            compute_node<T, Off, NDIM, SA, SB, SM>(M, k, mside, q);
+        }
+        return;
+    }
+    T col[CR];   // pcur of node m in slot (m - k_from + 1) % CR
+    T po[CR];    // pold of node m in slot (m - k_from) % CR
+    T fa[CR];    // fact of node m in slot (m - k_from) % CR
+    NodeLoads<T> nr[DN];  // near loads of node m in slot (m - k_from) % DN
+    // prologue, in steady-state order: near loads first, then far loads
+#pragma unroll
+    for (int u = 0; u < DN; ++u) load_near<T, Off, NDIM, SA, SB, SM>(M, k_from + u, mside, nr[u]);
+#pragma unroll
+    for (int u = 0; u < F + 2; ++u) col[u] = ldg(M.pcur, M.col + (Off)(k_from - 1 + u) * M.sm);
+#pragma unroll
+    for (int u = 0; u < F; ++u) {
+        po[u] = ldg_stream(M.pold, M.col + (Off)(k_from + u) * M.sm);
+        fa[u] = ldg(M.fact, M.col + (Off)(k_from + u) * M.sm);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    int k0 = k_from;
+    for (; k0 + CR + F <= k_to; k0 += CR) {
+#pragma unroll
+        for (int u = 0; u < CR; ++u) {
+            const int k = k0 + u;
+            NodeLoads<T>& q = nr[u % DN];
+            q.pm = col[u % CR];
+            q.pc = col[(u + 1) % CR];
+            q.pp = col[(u + 2) % CR];
+            q.po = po[u % CR];
+            q.f = fa[u % CR];
+            compute_node<T, Off, NDIM, SA, SB, SM>(M, k, mside, q);
+            load_near<T, Off, NDIM, SA, SB, SM>(M, k + DN, mside, nr[u % DN]);
+            col[(u + F + 2) % CR] = ldg(M.pcur, M.col + (Off)(k + F + 1) * M.sm);
+            po[(u + F) % CR] = ldg_stream(M.pold, M.col + (Off)(k + F) * M.sm);
+            fa[(u + F) % CR] = ldg(M.fact, M.col + (Off)(k + F) * M.sm);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    // tail: fewer than CR + F nodes, loads only inside the segment
+#pragma unroll
+    for (int u = 0; u < CR + F - 1; ++u) {
+        const int k = k0 + u;
+        if (k < k_to) {
+            NodeLoads<T>& q = nr[u % DN];
+            q.pm = col[u % CR];
+            q.pc = col[(u + 1) % CR];
+            q.pp = col[(u + 2) % CR];
+            q.po = po[u % CR];
+            q.f = fa[u % CR];
+            compute_node<T, Off, NDIM, SA, SB, SM>(M, k, mside, q);
+            if (k + DN < k_to) load_near<T, Off, NDIM, SA, SB, SM>(M, k + DN, mside, nr[u % DN]);
+            if (k + F + 1 <= k_to) col[(u + F + 2) % CR] = ldg(M.pcur, M.col + (Off)(k + F + 1) * M.sm);
+            if (k + F < k_to) {
+                po[(u + F) % CR] = ldg_stream(M.pold, M.col + (Off)(k + F) * M.sm);
+                fa[(u + F) % CR] = ldg(M.fact, M.col + (Off)(k + F) * M.sm);
+            }
+        }
+    }
+}
+
 // Split [kbeg, kend) into low-slab / interior / high-slab segments of the
 // march axis.
 template <typename T, typename Off, int NDIM, int D, bool SA, bool SB>
@@ -407,9 +544,15 @@ __device__ __forceinline__ void march_column(const March<T, Off, NDIM>& M, const
     const int lo_end = active_m[0] ? (int)min((int64_t)kend, halo + 1) : kbeg;
     const int hi_beg = active_m[1] ? (int)max((int64_t)kbeg, n_m - halo - 1) : kend;
     const int a = max(kbeg, lo_end), b = min(kend, max(a, hi_beg));
-    march_segment<T, Off, NDIM, D, SA, SB, true>(M, kbeg, min(lo_end, kend), 0);
-    march_segment<T, Off, NDIM, D, SA, SB, false>(M, a, b, 0);
-    march_segment<T, Off, NDIM, D, SA, SB, true>(M, max(b, hi_beg), kend, 1);
+    if constexpr (D >= 4) {
+        march_segment_far<T, Off, NDIM, D, SA, SB, true>(M, kbeg, min(lo_end, kend), 0);
+        march_segment_far<T, Off, NDIM, D, SA, SB, false>(M, a, b, 0);
+        march_segment_far<T, Off, NDIM, D, SA, SB, true>(M, max(b, hi_beg), kend, 1);
+    } else {
+        march_segment<T, Off, NDIM, D, SA, SB, true>(M, kbeg, min(lo_end, kend), 0);
+        march_segment<T, Off, NDIM, D, SA, SB, false>(M, a, b, 0);
+        march_segment<T, Off, NDIM, D, SA, SB, true>(M, max(b, hi_beg), kend, 1);
+    }
 }
 
 // Fill a ColSlab for an in-plane axis.  q: node index along the axis;

@@ -86,7 +86,7 @@ def time_variant(S, dtype, nshots, steps, warmup, freetop, kernel_launches=50, t
     # dominant kernel alone, HIP events on the plan's stream
     P.reset()
     ms = P.time_stencil(kernel_launches) / kernel_launches
-    tiling = P.tiling()
+    tiling = P.tiling() + (P.shot_group(),)
     P.close()
     B, N = algorithmic_bytes(S.n, S.halo, freetop, esize, nshots)
     pts = nshots * N * S.nt
@@ -95,7 +95,8 @@ def time_variant(S, dtype, nshots, steps, warmup, freetop, kernel_launches=50, t
         "shot_gpts_per_s": pts / t / 1e9,
         "effective_gbs": B * S.nt / t / 1e9,
         "kernel_ms": ms, "kernel_gbs": B / (ms * 1e-3) / 1e9,
-        "bytes_per_launch": B, "tiling": {"block_y": tiling[0], "march_len": tiling[1], "prefetch": tiling[2]},
+        "bytes_per_launch": B, "tiling": {"block_y": tiling[0], "march_len": tiling[1], "prefetch": tiling[2],
+                   "shot_group": tiling[3]},
         "_traces0": traces0,
     }
 
@@ -189,7 +190,8 @@ def bench_batch2d(steps):
     out = {"workload": f"2D {S.n[0]}^2 (1024^2 interior + 20 halo), {P.nshots} shots, nt={S.nt}, FP32",
            "s_per_solve": round(t, 4), "shot_gpts_per_s": round(P.nshots * N * S.nt / t / 1e9, 2),
            "kernel_ms": round(ms, 4), "kernel_gbs": round(B / ms / 1e6, 1),
-           "trace_rel_l2_vs_fp64_max": e, "tiling": [tune["block_y"], tune["march_len"], tune["prefetch"]]}
+           "trace_rel_l2_vs_fp64_max": e,
+           "tiling": [tune["block_y"], tune["march_len"], tune["prefetch"], tune["shot_group"]]}
     P.close()
     return out
 
@@ -296,12 +298,14 @@ def profile_kernel(S, dtype, nshots, tiling, launches):
     import torch
     P = build_plan(S, dtype, nshots)
     if tiling:
-        P.set_tiling(*tiling)
+        P.set_tiling(*tiling[:3])
+        if len(tiling) > 3:
+            P.set_shot_group(tiling[3])
     P.reset()
     ms = P.time_stencil(launches) / launches
     esize = 8 if dtype == "f64" else 4
     B, _ = algorithmic_bytes(S.n, S.halo, True, esize, nshots)
-    print(f"[profile] {dtype} S={nshots} tiling={P.tiling()} {launches} launches "
+    print(f"[profile] {dtype} S={nshots} tiling={P.tiling()} shot_group={P.shot_group()} {launches} launches "
           f"{ms:.4f} ms/launch, {B} algorithmic bytes/launch, {B / ms / 1e6:.1f} GB/s", flush=True)
     P.close()
     n = 1 << 28
@@ -327,7 +331,8 @@ def main():
     ap.add_argument("--profile-kernel", action="store_true",
                     help="profiler target: stencil launches of the FP32 S-shot variant only")
     ap.add_argument("--profile-dtype", default="f32", choices=["f32", "f64"])
-    ap.add_argument("--tiling", default="", help="block_y,march_len,prefetch for --profile-kernel")
+    ap.add_argument("--tiling", default="",
+                    help="block_y,march_len,prefetch[,shot_group] for --profile-kernel")
     ap.add_argument("--launches", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:

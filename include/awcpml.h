@@ -139,8 +139,10 @@ int awcpml_time_advance(awcpml_plan *plan, int64_t nsteps, double *ms_total);
 
 /* Launch geometry of the stencil kernel: wavefronts per block along y
  * (1, 2, 4 or 8), the length of the z (3D) or y (2D) march per block, and
- * the software-pipeline depth of the march (1..3: loads issued up to
- * prefetch-1 nodes ahead).  0 keeps the library's choice.  A block_y that
+ * the software pipeline of the march: 1..3 = whole-node slots issued up to
+ * prefetch-1 nodes ahead; 4, 5 = two-distance pipeline (HBM streams 3 or 4
+ * nodes ahead, cached neighbour loads 1 or 2 nodes ahead).  0 keeps the
+ * library's choice.  A block_y that
  * the current shot group does not divide resets the shot group to 1.  Used
  * by the autotuner; no reference counterpart (ParallelStencil picks its
  * launch shape itself). */

@@ -162,7 +162,7 @@ def test_every_tiling_is_exact():
     P = plan_from_oracle(o)
     for by in (1, 2, 4):
         for ml in (1, 3, 7, 33):
-            for pf in (1, 2, 3):
+            for pf in (1, 2, 3, 4, 5):
                 P.set_tiling(by, ml, pf)
                 P.reset()
                 P.advance(o.nt)
@@ -211,7 +211,7 @@ def test_shot_groups_are_exact(dim):
         S.close()
     M = plan_from_oracle(o, nshots=len(pos), pos_list=pos)
     for by, g in ((1, 1), (2, 2), (4, 2), (4, 4), (8, 1), (8, 4), (8, 8)):
-        for ml, pf in ((5, 2), (16, 3)):
+        for ml, pf in ((5, 2), (16, 3), (6, 4), (13, 5)):
             M.set_tiling(by, ml, pf)
             M.set_shot_group(g)
             assert M.shot_group() == g
