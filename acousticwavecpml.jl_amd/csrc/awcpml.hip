@@ -26,6 +26,7 @@
 
 #include "adjoint_kernels.h"
 #include "awcpml.h"
+#include "launch.h"
 #include "stencil_kernels.h"
 
 using awcpml::StencilArgs;
@@ -225,6 +226,15 @@ StencilArgs<T> make_args(const awcpml_plan* P, int rot, int par) {
     A.shot_group = G;
     A.nsgroups = (P->nshots + G - 1) / G;
     A.tiles_x = (int)(round_up(P->n[0], awcpml::kTileX) / awcpml::kTileX);
+    {
+        // lane-shuffle x slabs: each active side's psi layers 0..halo in one
+        // x tile, and the two sides' lane ranges disjoint
+        const int64_t nx = P->n[0], h = P->halo, t = awcpml::kTileX;
+        bool fast = h > 0 && nx >= 2 * h + 3;
+        if (P->active[0][0]) fast = fast && h < t;
+        if (P->active[0][1]) fast = fast && (nx - h - 2) / t == (nx - 2) / t;
+        A.x_fast = fast ? 1 : 0;
+    }
     if (P->ndim == 3) {
         A.tiles_y = (int)((P->n[1] - 2 + rows - 1) / rows);
         A.nchunks = (int)((P->n[2] - 2 + P->march_len - 1) / P->march_len);
@@ -238,38 +248,20 @@ StencilArgs<T> make_args(const awcpml_plan* P, int rot, int par) {
     return A;
 }
 
-template <typename T, typename Off, int D>
-void launch_variant(const awcpml_plan* P, const StencilArgs<T>& A) {
-    dim3 block(awcpml::kTileX, P->block_y);
-    dim3 grid(A.grid_blocks);
-    // march-axis CPML coefficients staged in LDS: 8 vectors of halo+1
-    const size_t lds = (size_t)8 * (P->halo + 1) * sizeof(T);
-    if (P->ndim == 3)
-        awcpml::stencil3d<T, Off, D><<<grid, block, lds, P->stream>>>(A);
-    else
-        awcpml::stencil2d<T, Off, D><<<grid, block, lds, P->stream>>>(A);
-}
-
-template <typename T, typename Off>
-void launch_depth(const awcpml_plan* P, const StencilArgs<T>& A) {
-    switch (P->prefetch) {
-        case 1: launch_variant<T, Off, 1>(P, A); break;
-        case 3: launch_variant<T, Off, 3>(P, A); break;
-        case 4: launch_variant<T, Off, 4>(P, A); break;
-        case 5: launch_variant<T, Off, 5>(P, A); break;
-        default: launch_variant<T, Off, 2>(P, A); break;
-    }
-}
-
 template <typename T>
 int launch_stencil_t(awcpml_plan* P) {
     StencilArgs<T> A = make_args<T>(P, P->rot, P->par);
+    awcpml::LaunchShape L;
+    L.grid = dim3(A.grid_blocks);
+    L.block = dim3(awcpml::kTileX, P->block_y);
+    // march-axis CPML coefficients staged in LDS: 8 vectors of halo+1
+    L.lds = (size_t)8 * (P->halo + 1) * sizeof(T);
+    L.stream = P->stream;
     // 32-bit byte offsets when one shot's field fits in 4 GiB (the slabs
     // are smaller than a field)
-    if ((uint64_t)P->field_elems * sizeof(T) < (1ull << 32))
-        launch_depth<T, uint32_t>(P, A);
-    else
-        launch_depth<T, uint64_t>(P, A);
+    const bool off64 = (uint64_t)P->field_elems * sizeof(T) >= (1ull << 32);
+    if (P->ndim == 3) awcpml::launch_plan_stencil<T, 3>(A, L, P->prefetch, off64);
+    else awcpml::launch_plan_stencil<T, 2>(A, L, P->prefetch, off64);
     HIP_TRY(hipGetLastError());
     return AWCPML_OK;
 }
@@ -512,16 +504,6 @@ int batch_build_t(awcpml_batch* B) {
     return AWCPML_OK;
 }
 
-template <typename T, typename Off, int D>
-void batch_launch_variant(const awcpml_batch* B, const awcpml::BatchArgs<T>& A) {
-    dim3 block(awcpml::kTileX, B->block_y);
-    const size_t lds = (size_t)8 * (B->halo_max + 1) * sizeof(T);
-    if (B->ndim == 3)
-        awcpml::stencil_batch<T, Off, D, 3><<<dim3(B->grid_blocks), block, lds, B->stream>>>(A);
-    else
-        awcpml::stencil_batch<T, Off, D, 2><<<dim3(B->grid_blocks), block, lds, B->stream>>>(A);
-}
-
 template <typename T>
 int batch_step_t(awcpml_batch* B, int64_t it0, bool with_io) {
     const int np = (int)B->plans.size();
@@ -533,23 +515,13 @@ int batch_step_t(awcpml_batch* B, int64_t it0, bool with_io) {
     A.grid_blocks = B->grid_blocks;
     for (int q = 0; q < np; ++q) A.nt[q] = B->plans[q]->nt;
     for (int q = 0; q <= np; ++q) A.block_start[q] = B->block_start[q];
-    if (B->use32) {
-        switch (B->prefetch) {
-            case 1: batch_launch_variant<T, uint32_t, 1>(B, A); break;
-            case 3: batch_launch_variant<T, uint32_t, 3>(B, A); break;
-            case 4: batch_launch_variant<T, uint32_t, 4>(B, A); break;
-            case 5: batch_launch_variant<T, uint32_t, 5>(B, A); break;
-            default: batch_launch_variant<T, uint32_t, 2>(B, A); break;
-        }
-    } else {
-        switch (B->prefetch) {
-            case 1: batch_launch_variant<T, uint64_t, 1>(B, A); break;
-            case 3: batch_launch_variant<T, uint64_t, 3>(B, A); break;
-            case 4: batch_launch_variant<T, uint64_t, 4>(B, A); break;
-            case 5: batch_launch_variant<T, uint64_t, 5>(B, A); break;
-            default: batch_launch_variant<T, uint64_t, 2>(B, A); break;
-        }
-    }
+    awcpml::LaunchShape L;
+    L.grid = dim3(B->grid_blocks);
+    L.block = dim3(awcpml::kTileX, B->block_y);
+    L.lds = (size_t)8 * (B->halo_max + 1) * sizeof(T);
+    L.stream = B->stream;
+    if (B->ndim == 3) awcpml::launch_batch_stencil<T, 3>(A, L, B->prefetch, !B->use32);
+    else awcpml::launch_batch_stencil<T, 2>(A, L, B->prefetch, !B->use32);
     HIP_TRY(hipGetLastError());
     if (with_io && B->has_io) {
         awcpml::IoBatchArgs<T> I;

@@ -1,0 +1,27 @@
+// launch.h -- host-side launchers of the stencil kernels.  Each (precision,
+// dimension, plan/batch) combination is instantiated in its own object file
+// (launch_kernels.hip, compiled once per combination by build.py), so the
+// heavily templated kernels compile in parallel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "stencil_kernels.h"
+
+namespace awcpml {
+
+struct LaunchShape {
+    dim3 grid, block;
+    size_t lds;          // dynamic LDS bytes (march-axis CPML coefficients)
+    hipStream_t stream;
+};
+
+// pipe: march pipeline (1..3 whole-node rings, 4..5 two-distance);
+// off64: 64-bit byte offsets (fields of 4 GiB and more).
+template <typename T, int NDIM>
+void launch_plan_stencil(const StencilArgs<T>& A, const LaunchShape& L, int pipe, bool off64);
+
+template <typename T, int NDIM>
+void launch_batch_stencil(const BatchArgs<T>& A, const LaunchShape& L, int pipe, bool off64);
+
+}  // namespace awcpml

@@ -1,0 +1,60 @@
+// launch_kernels.hip -- definitions of the launchers declared in launch.h,
+// instantiated for AWCPML_REAL (float or double), AWCPML_NDIM (2 or 3) and
+// AWCPML_BATCH (0: one plan, 1: a batch of plans), given on the compiler
+// command line (see build.py).
+#include "launch.h"
+
+#if !defined(AWCPML_REAL) || !defined(AWCPML_NDIM) || !defined(AWCPML_BATCH)
+#error "compile with -DAWCPML_REAL=<float|double> -DAWCPML_NDIM=<2|3> -DAWCPML_BATCH=<0|1>"
+#endif
+
+namespace awcpml {
+
+#if AWCPML_BATCH == 0
+template <typename T, typename Off, int P, int NDIM>
+static void go_plan(const StencilArgs<T>& A, const LaunchShape& L) {
+    if constexpr (NDIM == 3) stencil3d<T, Off, P><<<L.grid, L.block, L.lds, L.stream>>>(A);
+    else stencil2d<T, Off, P><<<L.grid, L.block, L.lds, L.stream>>>(A);
+}
+
+template <typename T, int NDIM>
+void launch_plan_stencil(const StencilArgs<T>& A, const LaunchShape& L, int pipe, bool off64) {
+    if (off64) {  // fields of 4 GiB and more: one pipeline shape
+        go_plan<T, uint64_t, 2, NDIM>(A, L);
+        return;
+    }
+    switch (pipe) {
+        case 1: go_plan<T, uint32_t, 1, NDIM>(A, L); break;
+        case 3: go_plan<T, uint32_t, 3, NDIM>(A, L); break;
+        case 4: go_plan<T, uint32_t, 4, NDIM>(A, L); break;
+        case 5: go_plan<T, uint32_t, 5, NDIM>(A, L); break;
+        default: go_plan<T, uint32_t, 2, NDIM>(A, L); break;
+    }
+}
+template void launch_plan_stencil<AWCPML_REAL, AWCPML_NDIM>(const StencilArgs<AWCPML_REAL>&,
+                                                           const LaunchShape&, int, bool);
+#else
+template <typename T, typename Off, int P, int NDIM>
+static void go_batch(const BatchArgs<T>& A, const LaunchShape& L) {
+    stencil_batch<T, Off, P, NDIM><<<L.grid, L.block, L.lds, L.stream>>>(A);
+}
+
+template <typename T, int NDIM>
+void launch_batch_stencil(const BatchArgs<T>& A, const LaunchShape& L, int pipe, bool off64) {
+    if (off64) {
+        go_batch<T, uint64_t, 2, NDIM>(A, L);
+        return;
+    }
+    switch (pipe) {
+        case 1: go_batch<T, uint32_t, 1, NDIM>(A, L); break;
+        case 3: go_batch<T, uint32_t, 3, NDIM>(A, L); break;
+        case 4: go_batch<T, uint32_t, 4, NDIM>(A, L); break;
+        case 5: go_batch<T, uint32_t, 5, NDIM>(A, L); break;
+        default: go_batch<T, uint32_t, 2, NDIM>(A, L); break;
+    }
+}
+template void launch_batch_stencil<AWCPML_REAL, AWCPML_NDIM>(const BatchArgs<AWCPML_REAL>&,
+                                                            const LaunchShape&, int, bool);
+#endif
+
+}  // namespace awcpml

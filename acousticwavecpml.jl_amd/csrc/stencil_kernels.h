@@ -85,6 +85,9 @@ struct StencilArgs {
     // (so the shared fact rows are fetched once per block); a block covers
     // blockDim.y / shot_group rows (2D: march chunks) of nsgroups groups
     int shot_group, nsgroups;
+    // 1: the x slabs use the lane-shuffle path (each active side's psi
+    // layers 0..halo lie in one 64-wide x tile, nx >= 2*halo+3)
+    int x_fast;
     int logical_blocks;
     int grid_blocks;          // launched blocks, a multiple of kNumXcd
     // optional: per node the derivative of pnew with respect to fact,
@@ -170,16 +173,16 @@ template <typename T, typename Off>
 __device__ __forceinline__ T cpml_apply(const LayerCoef<T>& c, T* psi_wr, T* xi_arr, Off psi_m,
                                         Off psi_m1, Off xi_m1, int m, T psi_old,
                                         T psi_old1, T xi_old, T pm, T pc, T pp, T d2,
-                                        T inv_d, T f) {
+                                        T inv_d, T f, bool st = true) {
     // update_psi_*!: psi[m] = b_K_h[m]*psi[m] + a_h[m]*(p[m+1]-p[m])*_d
     const T psi_here = c.bKh * psi_old + c.ah * (pp - pc) * inv_d;
     const T psi_prev = c.bKh1 * psi_old1 + c.ah1 * (pc - pm) * inv_d;
-    stg(psi_wr, psi_m, psi_here);
-    if (m == 1) stg(psi_wr, psi_m1, psi_prev);
+    if (st) stg(psi_wr, psi_m, psi_here);
+    if (st && m == 1) stg(psi_wr, psi_m1, psi_prev);
     // update_p!: dpsi, in-place xi, damping
     const T dpsi = (psi_here - psi_prev) * inv_d;
     const T xin = c.bK1 * xi_old + c.a1 * (d2 + dpsi);
-    stg(xi_arr, xi_m1, xin);
+    if (st) stg(xi_arr, xi_m1, xin);
     return dpsi + xin;
 }
 
@@ -208,7 +211,8 @@ __device__ __forceinline__ SlabPtrs<T> slab_ptrs(const StencilArgs<T>& A, int ax
 // xi[m-1] at march position 0 plus their increments per march step.
 template <typename T, typename Off>
 struct ColSlab {
-    bool in;            // this lane's column is in the slab
+    bool in;            // this lane's column is in the slab (1 <= m <= halo)
+    bool own;           // lane-shuffle x path: this lane computes psi[m], 0 <= m <= halo
     int m;
     SlabPtrs<T> p;
     LayerCoef<T> c;
@@ -278,9 +282,10 @@ struct March {
     const T* lds;             // march-axis coefficients
     int h1;
     T* qout;                  // optional d(pnew)/d(fact) output (this shot)
+    bool store_p;             // false on the x = 0 lane kept for the x-slab psi[0]
 };
 
-template <typename T, typename Off, int NDIM, bool SA, bool SB, bool SM>
+template <typename T, typename Off, int NDIM, int SA, bool SB, bool SM>
 __device__ __forceinline__ void load_node(const March<T, Off, NDIM>& M, int k, int mside,
                                           NodeLoads<T>& q) {
     constexpr Off E = (Off)sizeof(T);
@@ -296,7 +301,10 @@ __device__ __forceinline__ void load_node(const March<T, Off, NDIM>& M, int k, i
     }
     q.po = ldg_stream(M.pold, c);
     q.f = ldg(M.fact, c);
-    if (SA) {
+    if (SA == 1) {  // lane-shuffle path: psi[m] of the lane's layer, xi[m-1]
+        q.psa0 = ldg(M.sa.p.psi_rd, M.sa.o_psi + (Off)k * M.sa.st_psi);
+        q.xia = ldg((const T*)M.sa.p.xi, M.sa.o_xi + (Off)k * M.sa.st_xi);
+    } else if (SA == 2) {
         q.psa0 = ldg(M.sa.p.psi_rd, M.sa.o_psi + (Off)k * M.sa.st_psi);
         q.psa1 = ldg(M.sa.p.psi_rd, M.sa.o_psi1 + (Off)k * M.sa.st_psi);
         q.xia = ldg((const T*)M.sa.p.xi, M.sa.o_xi + (Off)k * M.sa.st_xi);
@@ -317,7 +325,7 @@ __device__ __forceinline__ void load_node(const March<T, Off, NDIM>& M, int k, i
     }
 }
 
-template <typename T, typename Off, int NDIM, bool SA, bool SB, bool SM>
+template <typename T, typename Off, int NDIM, int SA, bool SB, bool SM>
 __device__ __forceinline__ void compute_node(const March<T, Off, NDIM>& M, int k, int mside,
                                              const NodeLoads<T>& q) {
     const T two = T(2);
@@ -329,7 +337,24 @@ __device__ __forceinline__ void compute_node(const March<T, Off, NDIM>& M, int k
     const T d2m = (pp - two * pc + pm) * M.i2m;
     T damp = T(0);
     T slab = T(0);  // sum of (dpsi + xi) over the node's slabs, for qout
-    if (SA) {
+    // Only lane-shuffle x waves hold the x = 0 lane, which computes psi[0]
+    // of the low x side and stores nothing else.
+    const bool st = SA != 1 || M.store_p;
+    if (SA == 1) {
+        // psi[m] of this lane's layer; psi[m-1] is the same expression
+        // evaluated by lane i-1 (layer m-1), taken with a lane shift
+        const T ph = M.sa.c.bKh * q.psa0 + M.sa.c.ah * (q.xp - pc) * M.ia;
+        const T pprev = __shfl_up(ph, 1u, 64);
+        if (M.sa.own) stg(M.sa.p.psi_wr, M.sa.o_psi + (Off)k * M.sa.st_psi, ph);
+        if (M.sa.in) {
+            const T dpsi = (ph - pprev) * M.ia;
+            const T xin = M.sa.c.bK1 * q.xia + M.sa.c.a1 * (d2a + dpsi);
+            stg(M.sa.p.xi, M.sa.o_xi + (Off)k * M.sa.st_xi, xin);
+            const T t = dpsi + xin;
+            damp += f * t;
+            slab += t;
+        }
+    } else if (SA == 2) {
         if (M.sa.in) {
             const T t = cpml_apply(M.sa.c, M.sa.p.psi_wr, M.sa.p.xi, M.sa.o_psi + (Off)k * M.sa.st_psi,
                                    M.sa.o_psi1 + (Off)k * M.sa.st_psi, M.sa.o_xi + (Off)k * M.sa.st_xi,
@@ -341,7 +366,7 @@ __device__ __forceinline__ void compute_node(const March<T, Off, NDIM>& M, int k
     if (NDIM == 3 && SB) {
         const T t = cpml_apply(M.sb.c, M.sb.p.psi_wr, M.sb.p.xi, M.sb.o_psi + (Off)k * M.sb.st_psi,
                                M.sb.o_psi1 + (Off)k * M.sb.st_psi, M.sb.o_xi + (Off)k * M.sb.st_xi,
-                               M.sb.m, q.psb0, q.psb1, q.xib, q.ym, pc, q.yp, d2b, M.ib, f);
+                               M.sb.m, q.psb0, q.psb1, q.xib, q.ym, pc, q.yp, d2b, M.ib, f, st);
         damp += f * t;
         slab += t;
     }
@@ -351,7 +376,7 @@ __device__ __forceinline__ void compute_node(const March<T, Off, NDIM>& M, int k
         const SlabPtrs<T>& P = M.pm_side[mside];
         const T t = cpml_apply(cz, P.psi_wr, P.xi, M.col + (Off)mz * M.sm,
                                M.col + (Off)(mz - 1) * M.sm, M.col + (Off)(mz - 1) * M.sm, mz,
-                               q.psm0, q.psm1, q.xim, pm, pc, pp, d2m, M.imm, f);
+                               q.psm0, q.psm1, q.xim, pm, pc, pp, d2m, M.imm, f, st);
         damp += f * t;
         slab += t;
     }
@@ -359,14 +384,14 @@ __device__ __forceinline__ void compute_node(const March<T, Off, NDIM>& M, int k
     T lap;
     if (NDIM == 3) lap = d2a + d2b + d2m;   // (d2x + d2y) + d2z
     else lap = d2a + d2m;                   // d2x + d2y
-    stg_stream(M.pnew, M.col + (Off)k * M.sm, two * pc - q.po + f * lap + damp);
-    if (M.qout) stg(M.qout, M.col + (Off)k * M.sm, lap + slab);
+    if (st) stg_stream(M.pnew, M.col + (Off)k * M.sm, two * pc - q.po + f * lap + damp);
+    if (st && M.qout) stg(M.qout, M.col + (Off)k * M.sm, lap + slab);
 }
 
 // March over nodes [k_from, k_to) with a fixed load set.  Each node's slot
 // holds all its operands (pcur below/at/above included), so no value is
 // carried from one slot to the next and refills never need register copies.
-template <typename T, typename Off, int NDIM, int D, bool SA, bool SB, bool SM>
+template <typename T, typename Off, int NDIM, int D, int SA, bool SB, bool SM>
 __device__ __forceinline__ void march_segment(const March<T, Off, NDIM>& M, int k_from, int k_to,
                                               int mside) {
     if (k_to - k_from < 2 * D) {
@@ -437,7 +462,7 @@ struct Pipe<5> {
     static constexpr int CR = 6, F = 4, DN = 2;
 };
 
-template <typename T, typename Off, int NDIM, bool SA, bool SB, bool SM>
+template <typename T, typename Off, int NDIM, int SA, bool SB, bool SM>
 __device__ __forceinline__ void load_near(const March<T, Off, NDIM>& M, int k, int mside,
                                           NodeLoads<T>& q) {
     constexpr Off E = (Off)sizeof(T);
@@ -448,7 +473,10 @@ __device__ __forceinline__ void load_near(const March<T, Off, NDIM>& M, int k, i
         q.ym = ldg(M.pcur, c - M.sy);
         q.yp = ldg(M.pcur, c + M.sy);
     }
-    if (SA) {
+    if (SA == 1) {  // lane-shuffle path: psi[m] of the lane's layer, xi[m-1]
+        q.psa0 = ldg(M.sa.p.psi_rd, M.sa.o_psi + (Off)k * M.sa.st_psi);
+        q.xia = ldg((const T*)M.sa.p.xi, M.sa.o_xi + (Off)k * M.sa.st_xi);
+    } else if (SA == 2) {
         q.psa0 = ldg(M.sa.p.psi_rd, M.sa.o_psi + (Off)k * M.sa.st_psi);
         q.psa1 = ldg(M.sa.p.psi_rd, M.sa.o_psi1 + (Off)k * M.sa.st_psi);
         q.xia = ldg((const T*)M.sa.p.xi, M.sa.o_xi + (Off)k * M.sa.st_xi);
@@ -467,7 +495,7 @@ __device__ __forceinline__ void load_near(const March<T, Off, NDIM>& M, int k, i
     }
 }
 
-template <typename T, typename Off, int NDIM, int PK, bool SA, bool SB, bool SM>
+template <typename T, typename Off, int NDIM, int PK, int SA, bool SB, bool SM>
 __device__ __forceinline__ void march_segment_far(const March<T, Off, NDIM>& M, int k_from, int k_to,
                                                   int mside) {
     constexpr int CR = Pipe<PK>::CR, F = Pipe<PK>::F, DN = Pipe<PK>::DN;
@@ -538,7 +566,7 @@ __device__ __forceinline__ void march_segment_far(const March<T, Off, NDIM>& M, 
 
 // Split [kbeg, kend) into low-slab / interior / high-slab segments of the
 // march axis.
-template <typename T, typename Off, int NDIM, int D, bool SA, bool SB>
+template <typename T, typename Off, int NDIM, int D, int SA, bool SB>
 __device__ __forceinline__ void march_column(const March<T, Off, NDIM>& M, const int* active_m,
                                              int64_t n_m, int64_t halo, int kbeg, int kend) {
     const int lo_end = active_m[0] ? (int)min((int64_t)kend, halo + 1) : kbeg;
@@ -570,6 +598,7 @@ __device__ __forceinline__ void col_slab(const StencilArgs<T>& A, int axis, int6
         side = 0;
         m = 1;
     }
+    S.own = false;
     S.m = m;
     S.p = slab_ptrs(A, axis, side, shot);
     S.c = LayerCoef<T>{};
@@ -577,6 +606,50 @@ __device__ __forceinline__ void col_slab(const StencilArgs<T>& A, int axis, int6
     S.o_psi = psi_l0 + (Off)m * psi_dl;
     S.o_psi1 = psi_l0 + (Off)(m - 1) * psi_dl;
     S.o_xi = xi_l0 + (Off)(m - 1) * xi_dl;
+    S.st_psi = st_psi;
+    S.st_xi = st_xi;
+}
+
+// x slab for the lane-shuffle path: lanes q = 0..halo (low side) and
+// q = n-halo-2..n-2 (high side) own psi layer m = q - offset, 0..halo; the
+// lanes with m >= 1 are the slab nodes.  Same if/elseif order as
+// classify(); the sides cannot overlap because n >= 2*halo+3.
+template <typename T, typename Off>
+__device__ __forceinline__ void col_slab_xfast(const StencilArgs<T>& A, int64_t q, int64_t n,
+                                               int shot, Off psi_l0, Off psi_dl, Off xi_l0,
+                                               Off xi_dl, Off st_psi, Off st_xi, ColSlab<T, Off>& S) {
+    const int64_t h = A.halo;
+    int side = -1, m = 0;
+    if (q <= h) {
+        if (A.active[0][0]) {
+            side = 0;
+            m = (int)q;
+        }
+    } else if (q >= n - h - 2 && q <= n - 2 && A.active[0][1]) {
+        side = 1;
+        m = (int)(q - (n - h - 2));
+    }
+    S.own = side >= 0;
+    S.in = S.own && m >= 1;
+    if (!S.own) {
+        side = 0;
+        m = 1;  // a valid dummy location
+    }
+    S.m = m;
+    S.p = slab_ptrs(A, 0, side, shot);
+    S.c = LayerCoef<T>{};
+    if (S.own) {
+        const CpmlSide<T>& cs = A.side[0][side];
+        S.c.bKh = cs.bKh[m];
+        S.c.ah = cs.ah[m];
+        if (m >= 1) {
+            S.c.bK1 = cs.bK[m - 1];
+            S.c.a1 = cs.a[m - 1];
+        }
+    }
+    S.o_psi = psi_l0 + (Off)m * psi_dl;
+    S.o_psi1 = S.o_psi;
+    S.o_xi = xi_l0 + (Off)(m >= 1 ? m - 1 : 0) * xi_dl;
     S.st_psi = st_psi;
     S.st_xi = st_xi;
 }
@@ -606,7 +679,8 @@ __device__ __forceinline__ void stencil3d_block(const StencilArgs<T>& A, int lb,
     const int64_t nx = A.nx, ny = A.ny, nz = A.nz, halo = A.halo;
     if (shot >= A.nshots) return;
     if (j > ny - 2) return;
-    if (i < 1 || i > nx - 2) return;
+    // the x = 0 lane stays when it computes psi[0] of the low x side
+    if (i < ((A.x_fast && A.active[0][0]) ? 0 : 1) || i > nx - 2) return;
     const int kbeg = 1 + chunk * A.march_len;
     const int kend = (int)min((int64_t)kbeg + A.march_len, nz - 1);
     if (kbeg >= kend) return;
@@ -631,9 +705,14 @@ __device__ __forceinline__ void stencil3d_block(const StencilArgs<T>& A, int lb,
     M.lds = lds;
     M.h1 = (int)halo + 1;
     const Off hp1 = (Off)halo + 1, hl = (Off)halo;
+    M.store_p = i >= 1;
     // psi_x: (halo+1) x ny x nz, xi_x: halo x ny x nz
-    col_slab<T, Off>(A, 0, i, nx, shot, hp1 * (Off)j * E, E, hl * (Off)j * E, E,
-                     hp1 * (Off)ny * E, hl * (Off)ny * E, M.sa);
+    if (A.x_fast)
+        col_slab_xfast<T, Off>(A, i, nx, shot, hp1 * (Off)j * E, E, hl * (Off)j * E, E,
+                               hp1 * (Off)ny * E, hl * (Off)ny * E, M.sa);
+    else
+        col_slab<T, Off>(A, 0, i, nx, shot, hp1 * (Off)j * E, E, hl * (Off)j * E, E,
+                         hp1 * (Off)ny * E, hl * (Off)ny * E, M.sa);
     // psi_y: px x (halo+1) x nz, xi_y: px x halo x nz
     col_slab<T, Off>(A, 1, j, ny, shot, (Off)i * E, M.sy, (Off)i * E, M.sy, hp1 * M.sy,
                      hl * M.sy, M.sb);
@@ -642,14 +721,17 @@ __device__ __forceinline__ void stencil3d_block(const StencilArgs<T>& A, int lb,
     M.mshift[0] = 0;
     M.mshift[1] = (int)(nz - halo - 2);
 
-    const bool xs = __any(M.sa.in);
+    const bool xs = A.x_fast ? __any(M.sa.own) : __any(M.sa.in);
     const bool ys = M.sb.in;  // wave-uniform
-    if (xs) {
-        if (ys) march_column<T, Off, 3, D, true, true>(M, A.active[2], nz, halo, kbeg, kend);
-        else march_column<T, Off, 3, D, true, false>(M, A.active[2], nz, halo, kbeg, kend);
+    if (xs && A.x_fast) {
+        if (ys) march_column<T, Off, 3, D, 1, true>(M, A.active[2], nz, halo, kbeg, kend);
+        else march_column<T, Off, 3, D, 1, false>(M, A.active[2], nz, halo, kbeg, kend);
+    } else if (xs) {
+        if (ys) march_column<T, Off, 3, D, 2, true>(M, A.active[2], nz, halo, kbeg, kend);
+        else march_column<T, Off, 3, D, 2, false>(M, A.active[2], nz, halo, kbeg, kend);
     } else {
-        if (ys) march_column<T, Off, 3, D, false, true>(M, A.active[2], nz, halo, kbeg, kend);
-        else march_column<T, Off, 3, D, false, false>(M, A.active[2], nz, halo, kbeg, kend);
+        if (ys) march_column<T, Off, 3, D, 0, true>(M, A.active[2], nz, halo, kbeg, kend);
+        else march_column<T, Off, 3, D, 0, false>(M, A.active[2], nz, halo, kbeg, kend);
     }
 }
 
@@ -673,7 +755,7 @@ __device__ __forceinline__ void stencil2d_block(const StencilArgs<T>& A, int lb,
     const int64_t nx = A.nx, ny = A.ny, halo = A.halo;
     if (shot >= A.nshots) return;
     if (chunk >= A.nchunks) return;
-    if (i < 1 || i > nx - 2) return;
+    if (i < ((A.x_fast && A.active[0][0]) ? 0 : 1) || i > nx - 2) return;
     const int jbeg = 1 + chunk * A.march_len;
     const int jend = (int)min((int64_t)jbeg + A.march_len, ny - 1);
     if (jbeg >= jend) return;
@@ -698,8 +780,10 @@ __device__ __forceinline__ void stencil2d_block(const StencilArgs<T>& A, int lb,
     M.lds = lds;
     M.h1 = (int)halo + 1;
     const Off hp1 = (Off)halo + 1, hl = (Off)halo;
+    M.store_p = i >= 1;
     // psi_x: (halo+1) x ny, xi_x: halo x ny
-    col_slab<T, Off>(A, 0, i, nx, shot, 0, E, 0, E, hp1 * E, hl * E, M.sa);
+    if (A.x_fast) col_slab_xfast<T, Off>(A, i, nx, shot, 0, E, 0, E, hp1 * E, hl * E, M.sa);
+    else col_slab<T, Off>(A, 0, i, nx, shot, 0, E, 0, E, hp1 * E, hl * E, M.sa);
     M.sb = M.sa;
     M.sb.in = false;
     // psi_y: px x (halo+1), xi_y: px x halo
@@ -707,8 +791,12 @@ __device__ __forceinline__ void stencil2d_block(const StencilArgs<T>& A, int lb,
     M.mshift[0] = 0;
     M.mshift[1] = (int)(ny - halo - 2);
 
-    if (__any(M.sa.in)) march_column<T, Off, 2, D, true, false>(M, A.active[1], ny, halo, jbeg, jend);
-    else march_column<T, Off, 2, D, false, false>(M, A.active[1], ny, halo, jbeg, jend);
+    if (A.x_fast && __any(M.sa.own))
+        march_column<T, Off, 2, D, 1, false>(M, A.active[1], ny, halo, jbeg, jend);
+    else if (!A.x_fast && __any(M.sa.in))
+        march_column<T, Off, 2, D, 2, false>(M, A.active[1], ny, halo, jbeg, jend);
+    else
+        march_column<T, Off, 2, D, 0, false>(M, A.active[1], ny, halo, jbeg, jend);
 }
 
 // ---------------------------------------------------------------------

@@ -132,3 +132,42 @@ def test_fp32_traces_3d_small():
     o = run_oracle(case)
     tr, _ = run_gpu(o, dtype="f32")
     assert rel_l2(tr, o.traces()) <= FP32_TRACE_TOL
+
+
+def _case_2d_narrow():
+    """nx = 2*halo + 2: the two x slabs touch, so the x slabs take the
+    per-lane path instead of the lane-shuffle one."""
+    c = small2d(n=(14, 40), halo=6, nt=120, freetop=False)
+    dx = 10.0
+    c["srcpos"] = np.array([[7 * dx, 12 * dx], [3 * dx, 0.0]])
+    c["recpos"] = np.array([[(2 + r) * dx, 8 * dx] for r in range(10)] + [[0.0, 20 * dx]])
+    return c
+
+
+@pytest.mark.parametrize("name", ["3d_straddle", "2d_straddle", "2d_narrow", "3d_fast"])
+def test_x_slab_paths_bitwise(name):
+    """Both x-slab code paths against the oracle: grids whose high x slab
+    crosses a 64-wide tile boundary or whose slabs touch use the per-lane
+    path, the others the lane-shuffle path."""
+    case = {"3d_straddle": lambda: small3d(n=(67, 21, 19), halo=6, nt=50, freetop=False),
+            "2d_straddle": lambda: small2d(n=(70, 30), halo=6, nt=120),
+            "2d_narrow": _case_2d_narrow,
+            "3d_fast": lambda: small3d(n=(60, 23, 21), halo=8, nt=50)}[name]()
+    print(f"[gpu] x slab paths {name}", flush=True)
+    o = oracle_problem(case)
+    o.run()
+    sp = [case["l"][a] / (case["n"][a] - 1) for a in range(len(case["n"]))]
+    P = awcpml.Plan(o.n, o.halo, dtype="f64")
+    P.set_model(o.fact(), [1 / d for d in sp], [1 / (d * d) for d in sp])
+    for a in range(o.ndim):
+        P.set_cpml(a, [o.coef(a, w) for w in range(8)])
+    ps, pr = o.positions()
+    P.set_sources(0, ps, o.dt2srctf())
+    P.set_receivers(0, pr)
+    for by, ml, pf in ((4, 7, 2), (1, 16, 3), (2, 40, 5)):
+        P.set_tiling(by, ml, pf)
+        P.reset()
+        P.advance(o.nt)
+        assert np.array_equal(P.traces(), o.traces()), (by, ml, pf)
+        assert np.array_equal(P.field(), o.pend()), (by, ml, pf)
+    P.close()
