@@ -30,6 +30,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "stencil_kernels.h"  // kTileX, logical_block
+
 namespace awcpml {
 
 template <typename T>
@@ -64,9 +66,9 @@ struct AdjointArgs {
     int64_t xi_stride[3];
     int ndim;
     int nshots;
-    int64_t columns;          // nodes per shot handled by one thread's march
-    int march_len, nchunks;
-    int64_t total_threads;
+    int march_len, nchunks;   // march (z in 3D, y in 2D) chunking
+    int tiles_x, tiles_y;     // 64-wide x tiles; 3D: row groups, 2D: chunk groups
+    int logical_blocks, grid_blocks;
 };
 
 // Element index of layer L of axis a's slab at node (i, j, k) in a slab
@@ -78,28 +80,20 @@ __device__ __forceinline__ int64_t slab_index(int a, int64_t layers, int64_t L, 
     return i + px * (j + ny * L);
 }
 
-// One node's adjoint update; (i, j, k) interior, lin its element offset.
+// One node's adjoint update.  (i, j, k) interior (2D: j is the march index
+// and k = 0), lin its element offset in the shot's field; fc, lc: fact and
+// lam^{s+1} at the node; um/up: f lam^{s+1} one node below/above along each
+// axis; l2, qv, gv: lam^{s+2}, q^s and the gradient at the node.
 template <typename T, int NDIM>
-__device__ __forceinline__ void adjoint_node(const AdjointArgs<T>& A, int shot, int64_t i,
-                                             int64_t j, int64_t k) {
+__device__ __forceinline__ void adjoint_core(const AdjointArgs<T>& A, int shot, int64_t i,
+                                             int64_t j, int64_t k, int64_t lin, int64_t fs,
+                                             T fc, T lc, const T* um, const T* up, T l2, T qv,
+                                             T gv) {
     const int64_t px = A.px, ny = A.n[1];
-    const int64_t fs = (int64_t)shot * A.field_stride;
-    const int64_t lin = i + px * (j + ny * k);
-    const T* l1 = A.lam1 + fs;
-    const T* f = A.fact;
-    const int64_t stride[3] = {1, px, px * ny};
-    const T fc = f[lin];
-    const T lc = l1[lin];
     const T uc = fc * lc;
-    T acc = T(2) * lc - A.lam2[fs + lin];
-    T um[3], up[3];
+    T acc = T(2) * lc - l2;
 #pragma unroll
-    for (int a = 0; a < NDIM; ++a) {
-        const int64_t sa = stride[a];
-        um[a] = f[lin - sa] * l1[lin - sa];
-        up[a] = f[lin + sa] * l1[lin + sa];
-        acc += (up[a] - T(2) * uc + um[a]) * A.inv_d2[a];
-    }
+    for (int a = 0; a < NDIM; ++a) acc += (up[a] - T(2) * uc + um[a]) * A.inv_d2[a];
     const int64_t h = A.halo;
     const int64_t c[3] = {i, j, k};
     // slab sides of this node whose beta must be advanced after lam^s
@@ -165,36 +159,106 @@ __device__ __forceinline__ void adjoint_node(const AdjointArgs<T>& A, int shot, 
         const AdjSide<T>& S = A.side[a][s];
         S.be_wr[shot * A.xi_stride[a] + bs_idx[b]] = S.bK[bs_layer[b]] * bs_old[b] + fc * acc;
     }
-    A.grad[fs + lin] += acc * A.q[fs + lin];
+    A.grad[fs + lin] = gv + acc * qv;
 }
 
-// One thread per (x, y) column (3D) or x column (2D) of one shot, marching
-// over a chunk of march_len nodes along the last axis; x is the fastest
-// thread index so a wavefront reads 64 consecutive nodes of a row.
+// Loads of one node that are not carried along the march: the in-plane
+// neighbours of lam^{s+1} and fact, lam^{s+2}, q^s and the gradient.
+template <typename T>
+struct AdjNear {
+    T lxm, lxp, fxm, fxp, lym, lyp, fym, fyp, l2, qv, gv;
+};
+
+template <typename T, int NDIM>
+__device__ __forceinline__ void adj_load_near(const T* l1, const T* f, const T* l2, const T* q,
+                                              const T* g, int64_t lin, int64_t sy, AdjNear<T>& n) {
+    n.lxm = l1[lin - 1];
+    n.lxp = l1[lin + 1];
+    n.fxm = f[lin - 1];
+    n.fxp = f[lin + 1];
+    if (NDIM == 3) {
+        n.lym = l1[lin - sy];
+        n.lyp = l1[lin + sy];
+        n.fym = f[lin - sy];
+        n.fyp = f[lin + sy];
+    }
+    n.l2 = l2[lin];
+    n.qv = q[lin];
+    n.gv = g[lin];
+}
+
+// One reverse step.  Block (64, BY) as the forward kernel: lane -> x,
+// threadIdx.y -> y row (3D) or march chunk (2D); each thread marches a
+// chunk of march_len nodes along the last axis.  f lam^{s+1} along the
+// march is carried in registers (one new plane per node), and every load
+// of node k+1 is issued before node k is computed.  Logical blocks: x tile
+// fastest, then row/chunk group, chunk (3D), shot, renumbered XCD-aware.
 template <typename T, int NDIM>
 __global__ __launch_bounds__(256) void adjoint_step(const AdjointArgs<T> A) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= A.total_threads) return;
-    const int64_t nx = A.n[0], ny = A.n[1], nz = A.n[2];
-    // t = ((shot * nchunks + chunk) * rows + row) * (nx - 2) + (i - 1)
-    const int64_t rowlen = nx - 2;
-    const int64_t rows = (NDIM == 3) ? (ny - 2) : 1;
-    int64_t r = t;
-    const int64_t i = 1 + r % rowlen;
-    r /= rowlen;
-    const int64_t row = r % rows;
-    r /= rows;
-    const int chunk = (int)(r % A.nchunks);
-    const int shot = (int)(r / A.nchunks);
-    const int64_t mext = (NDIM == 3) ? nz : ny;
-    const int64_t k0 = 1 + (int64_t)chunk * A.march_len;
-    int64_t k1 = k0 + A.march_len;
-    if (k1 > mext - 1) k1 = mext - 1;
+    const int lb = logical_block(blockIdx.x, A.grid_blocks);
+    if (lb >= A.logical_blocks) return;
+    const int64_t nx = A.n[0], ny = A.n[1], nz = A.n[2], px = A.px;
+    int rest = lb;
+    const int tx = rest % A.tiles_x;
+    rest /= A.tiles_x;
+    const int ty = rest % A.tiles_y;
+    rest /= A.tiles_y;
+    const int wy = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    int chunk, shot;
+    int64_t j = 0;
     if (NDIM == 3) {
-        const int64_t j = 1 + row;
-        for (int64_t k = k0; k < k1; ++k) adjoint_node<T, 3>(A, shot, i, j, k);
+        chunk = rest % A.nchunks;
+        shot = rest / A.nchunks;
+        j = 1 + (int64_t)ty * blockDim.y + wy;
+        if (j > ny - 2) return;
     } else {
-        for (int64_t k = k0; k < k1; ++k) adjoint_node<T, 2>(A, shot, i, k, 0);
+        shot = rest;
+        chunk = ty * (int)blockDim.y + wy;
+        if (chunk >= A.nchunks) return;
+    }
+    const int64_t i = (int64_t)tx * kTileX + threadIdx.x;
+    if (i < 1 || i > nx - 2) return;
+    const int64_t mext = NDIM == 3 ? nz : ny;
+    const int64_t k0 = 1 + (int64_t)chunk * A.march_len;
+    const int64_t k1 = k0 + A.march_len < mext - 1 ? k0 + A.march_len : mext - 1;
+    if (k0 >= k1) return;
+    const int64_t ms = NDIM == 3 ? px * ny : px;  // march stride
+    const int64_t fs = (int64_t)shot * A.field_stride;
+    const T* l1 = A.lam1 + fs;
+    const T* l2 = A.lam2 + fs;
+    const T* qq = A.q + fs;
+    const T* gg = A.grad + fs;
+    const T* f = A.fact;
+    const int64_t base = NDIM == 3 ? i + px * j : i;
+    int64_t lin = base + k0 * ms;
+    T u_m = f[lin - ms] * l1[lin - ms];
+    T lc = l1[lin], fc = f[lin];
+    T lp = l1[lin + ms], fp = f[lin + ms];
+    AdjNear<T> nr;
+    adj_load_near<T, NDIM>(l1, f, l2, qq, gg, lin, px, nr);
+    for (int64_t k = k0; k < k1; ++k, lin += ms) {
+        // next node's loads (planes stay inside the field at the chunk end)
+        const int64_t kk = k + 2 < mext ? k + 2 : mext - 1;
+        const T lpp = l1[base + kk * ms], fpp = f[base + kk * ms];
+        AdjNear<T> nn;
+        adj_load_near<T, NDIM>(l1, f, l2, qq, gg, lin + ms, px, nn);
+        T um[3], up[3];
+        um[0] = nr.fxm * nr.lxm;
+        up[0] = nr.fxp * nr.lxp;
+        if (NDIM == 3) {
+            um[1] = nr.fym * nr.lym;
+            up[1] = nr.fyp * nr.lyp;
+        }
+        um[NDIM - 1] = u_m;
+        up[NDIM - 1] = fp * lp;
+        if (NDIM == 3) adjoint_core<T, 3>(A, shot, i, j, k, lin, fs, fc, lc, um, up, nr.l2, nr.qv, nr.gv);
+        else adjoint_core<T, 2>(A, shot, i, k, 0, lin, fs, fc, lc, um, up, nr.l2, nr.qv, nr.gv);
+        u_m = fc * lc;
+        lc = lp;
+        fc = fp;
+        lp = lpp;
+        fp = fpp;
+        nr = nn;
     }
 }
 

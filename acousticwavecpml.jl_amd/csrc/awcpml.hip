@@ -1153,6 +1153,8 @@ int awcpml_get_tiling(const awcpml_plan* P, int* block_y, int* march_len, int* p
 // ---------------------------------------------------------------------
 namespace {
 
+constexpr int kAdjBlockY = 4;  // wavefronts per block of the reverse step
+
 struct CkLayout {
     size_t field;            // bytes of one field buffer (all shots)
     size_t psi[3], xi[3];    // bytes of one side's slab (all shots)
@@ -1278,11 +1280,18 @@ awcpml::AdjointArgs<T> make_adj_args(const awcpml_plan* P, const AdjBuffers& Bf,
     }
     A.ndim = P->ndim;
     A.nshots = P->nshots;
-    A.march_len = 16;
+    A.march_len = 32;
     const int64_t mext = P->ndim == 3 ? P->n[2] : P->n[1];
     A.nchunks = (int)((mext - 2 + A.march_len - 1) / A.march_len);
-    const int64_t rows = P->ndim == 3 ? P->n[1] - 2 : 1;
-    A.total_threads = (int64_t)P->nshots * A.nchunks * rows * (P->n[0] - 2);
+    A.tiles_x = (int)(round_up(P->n[0], awcpml::kTileX) / awcpml::kTileX);
+    if (P->ndim == 3) {
+        A.tiles_y = (int)((P->n[1] - 2 + kAdjBlockY - 1) / kAdjBlockY);
+        A.logical_blocks = A.tiles_x * A.tiles_y * A.nchunks * P->nshots;
+    } else {
+        A.tiles_y = (A.nchunks + kAdjBlockY - 1) / kAdjBlockY;
+        A.logical_blocks = A.tiles_x * A.tiles_y * P->nshots;
+    }
+    A.grid_blocks = (int)round_up(A.logical_blocks, awcpml::kNumXcd);
     return A;
 }
 
@@ -1383,11 +1392,11 @@ int adjoint_t(awcpml_plan* P, const double* tbar, double* fbar) {
         for (int64_t s = c1; s > c0; --s) {
             const void* q = (const char*)Bf.q + (size_t)(s - c0 - 1) * fb;
             awcpml::AdjointArgs<T> A = make_adj_args<T>(P, Bf, lr, par, q);
-            const int64_t nb = (A.total_threads + 255) / 256;
+            const dim3 grid(A.grid_blocks), block(awcpml::kTileX, kAdjBlockY);
             if (P->ndim == 3)
-                awcpml::adjoint_step<T, 3><<<dim3((unsigned)nb), dim3(256), 0, P->stream>>>(A);
+                awcpml::adjoint_step<T, 3><<<grid, block, 0, P->stream>>>(A);
             else
-                awcpml::adjoint_step<T, 2><<<dim3((unsigned)nb), dim3(256), 0, P->stream>>>(A);
+                awcpml::adjoint_step<T, 2><<<grid, block, 0, P->stream>>>(A);
             HIP_TRY(hipGetLastError());
             if (!node_lin.empty()) {
                 io.lam0 = A.lam0;

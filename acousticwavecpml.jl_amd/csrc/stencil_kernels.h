@@ -664,11 +664,15 @@ __device__ __forceinline__ void stencil3d_block(const StencilArgs<T>& A, int lb,
     if (A.active[2][0] || A.active[2][1]) stage_march_coef(A, 2, lds);
     __syncthreads();
 
+    // logical order: x tile fastest, then y tile, chunk, shot group, so the
+    // blocks resident on one XCD at a time cover whole planes of one shot
+    // (their halo rows and lines are shared through that XCD's L2); with
+    // the shot fastest instead, an 8-shot launch moved ~10 % more bytes
     int rest = lb;
-    const int sg = rest % A.nsgroups; rest /= A.nsgroups;
     const int tx = rest % A.tiles_x; rest /= A.tiles_x;
-    const int ty = rest % A.tiles_y;
-    const int chunk = rest / A.tiles_y;
+    const int ty = rest % A.tiles_y; rest /= A.tiles_y;
+    const int chunk = rest % A.nchunks;
+    const int sg = rest / A.nchunks;
     // blockDim.x == 64 == wavefront: threadIdx.y is the same on all lanes
     const int wy = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     const int G = A.shot_group;
@@ -743,10 +747,10 @@ __device__ __forceinline__ void stencil2d_block(const StencilArgs<T>& A, int lb,
     if (A.active[1][0] || A.active[1][1]) stage_march_coef(A, 1, lds);
     __syncthreads();
 
-    int rest = lb;
-    const int sg = rest % A.nsgroups; rest /= A.nsgroups;
+    int rest = lb;  // x tile fastest, then chunk group, shot group
     const int tx = rest % A.tiles_x; rest /= A.tiles_x;
-    const int ty = rest;  // group of blockDim.y / shot_group chunks
+    const int ty = rest % A.tiles_y;  // group of blockDim.y / shot_group chunks
+    const int sg = rest / A.tiles_y;
     const int wy = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     const int G = A.shot_group;
     const int shot = sg * G + wy % G;
