@@ -242,6 +242,8 @@ __global__ __launch_bounds__(256) void adjoint_step(const AdjointArgs<T> A) {
         const T lpp = l1[base + kk * ms], fpp = f[base + kk * ms];
         AdjNear<T> nn;
         adj_load_near<T, NDIM>(l1, f, l2, qq, gg, lin + ms, px, nn);
+        // keep the next node's loads ahead of this node's arithmetic
+        __builtin_amdgcn_sched_barrier(0);
         T um[3], up[3];
         um[0] = nr.fxm * nr.lxm;
         up[0] = nr.fxp * nr.lxp;

@@ -334,6 +334,8 @@ def main():
     ap.add_argument("--tiling", default="",
                     help="block_y,march_len,prefetch[,shot_group] for --profile-kernel")
     ap.add_argument("--launches", type=int, default=20)
+    ap.add_argument("--profile-fwi", action="store_true",
+                    help="profiler target: one FP32 FWI gradient of the 554^2 32-shot workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -344,6 +346,22 @@ def main():
     from awcpml.host import setup
     import awcpml
     from cases import plain3d, rel_l2
+
+    if args.profile_fwi:
+        from awcpml import fwi
+        from cases import fwi2d
+        fc = fwi2d()
+        FS = _setup(fc)
+        P = _shot_plan(FS, fc["shots"], "f32")
+        P.reset()
+        P.advance(FS.nt)
+        obs = [P.traces(q) for q in range(P.nshots)]
+        P.set_model(np.asfortranarray((FS.dt * FS.dt) * (fc["start"] * fc["start"])), FS.inv_d, FS.inv_d2)
+        t0 = time.perf_counter()
+        J, g = fwi.misfit_gradient(P, obs, every=fc["every"])
+        print(f"[profile] fwi2d FP32 gradient {time.perf_counter() - t0:.3f} s, misfit {J:.4e}", flush=True)
+        P.close()
+        return
 
     case = plain3d(nt=args.nt)
     srcs = awcpml.Sources(case["srcpos"], case["t0s"], [awcpml.rickersource1D], case["f0"])
