@@ -11,9 +11,9 @@ Public surface (mirrors the reference's exports and solver entry points):
 """
 from .host import Receivers, Sources, calc_Kab_CPML, rickersource1D, setup  # noqa: F401
 from .solver import Batch, Plan, solve2D, solve3D  # noqa: F401
-from .autotune import autotune  # noqa: F401
+from .autotune import autotune, autotune_batch  # noqa: F401
 from . import _lib  # noqa: F401
 from . import fwi  # noqa: F401
 
 __all__ = ["Sources", "Receivers", "rickersource1D", "calc_Kab_CPML", "setup", "Plan",
-           "Batch", "autotune", "solve2D", "solve3D", "fwi"]
+           "Batch", "autotune", "autotune_batch", "solve2D", "solve3D", "fwi"]

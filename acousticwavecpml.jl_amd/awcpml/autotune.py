@@ -83,3 +83,29 @@ def autotune(plan, cands=None, launches=8, cache=DEFAULT_CACHE, use_cache=True):
         except OSError:
             pass
     return dict(best, cached=False)
+
+
+def autotune_batch(plans, cands=None, launches=3):
+    """Pick the launch shape (block_y, march_len, prefetch) shared by a batch
+    of plans: each candidate is applied to every plan, a Batch is built and
+    its stencil launch timed.  Returns the best shape and leaves it applied
+    to the plans (build the Batch afterwards)."""
+    from .solver import Batch
+    cands = cands or [(by, ml, pf) for by in (2, 4) for ml in (32, 64, 128) for pf in (2, 3, 5)]
+    results = []
+    for by, ml, pf in cands:
+        for P in plans:
+            P.set_tiling(by, ml, pf)
+            P.set_shot_group(1)
+        B = Batch(plans)
+        B.reset()
+        B.time_stencil(2)
+        ms = B.time_stencil(launches) / launches
+        B.close()
+        results.append((ms, by, ml, pf))
+    ms, by, ml, pf = min(results)
+    for P in plans:
+        P.set_tiling(by, ml, pf)
+        P.reset()
+    return {"block_y": by, "march_len": ml, "prefetch": pf, "ms_per_launch": ms,
+            "tried": len(results)}

@@ -212,6 +212,7 @@ def bench_uneven3d():
         P = awcpml.Plan.from_setup(S, dtype="f32")
         plans.append(P)
         setups.append(S)
+    tune = awcpml.autotune_batch(plans)
     B = awcpml.Batch(plans)
     B.set_graph(True)
     B.run()
@@ -236,7 +237,8 @@ def bench_uneven3d():
                         f"each, nt={setups[0].nt}, FP32, one batch launch per step",
             "s_per_solve": round(t, 4), "shot_gpts_per_s": round(pts / t / 1e9, 2),
             "kernel_ms": round(ms, 4), "kernel_gbs": round(bytes_launch / ms / 1e6, 1),
-            "trace_rel_l2_vs_fp64": [float("%.3g" % e) for e in errs]}
+            "trace_rel_l2_vs_fp64": [float("%.3g" % e) for e in errs],
+            "tiling": [tune["block_y"], tune["march_len"], tune["prefetch"]]}
 
 
 def bench_fwi2d():

@@ -224,3 +224,21 @@ def test_shot_groups_are_exact(dim):
         M.set_tiling(2, 8, 2)
         M.set_shot_group(4)
     M.close()
+
+
+def test_autotune_batch_keeps_results():
+    cases = [small3d(n=(37, 29, 23), halo=5, nt=40, seed=1),
+             small3d(n=(45, 41, 33), halo=6, nt=40, seed=2, freetop=False)]
+    print("[gpu] autotune batch", flush=True)
+    oracles = [oracle_for(c) for c in cases]
+    plans = [plan_from_oracle(o) for o in oracles]
+    best = awcpml.autotune_batch(plans, launches=1)
+    assert best["tried"] == 18
+    assert all(P.tiling() == (best["block_y"], best["march_len"], best["prefetch"]) for P in plans)
+    B = awcpml.Batch(plans)
+    B.run()
+    for P, o in zip(plans, oracles):
+        o.run()
+        assert np.array_equal(P.traces(), o.traces())
+        P.close()
+    B.close()
