@@ -80,12 +80,98 @@ __device__ __forceinline__ int64_t slab_index(int a, int64_t layers, int64_t L, 
     return i + px * (j + ny * L);
 }
 
+// x-axis CPML data of one lane (constant along the march): whether the
+// lane's column lies in the range 0 <= m <= halo+1 of an active x side,
+// the coefficients of the layers and edges it touches, and the offsets of
+// its alpha/beta rows.  Only the slab values themselves are loaded per node.
+template <typename T>
+struct AdjX {
+    bool ext;
+    int m;
+    bool v0, v1, v2;           // mm = m-1, m, m+1 are slab layers (1..halo)
+    T a0, a1, a2;              // a[mm-1] of those layers
+    T bK1;                     // bK[m-1] (the node's own beta)
+    bool e_lo, e_hi, w_lo;     // edges m-1 / m exist; this lane writes alpha[m-1]
+    T bKh_lo, ah_lo, bKh_hi, ah_hi;
+    const T* be_rd;
+    T* be_wr;
+    const T* al_rd;
+    T* al_wr;
+    int64_t be0, al0;          // element offset of layer 0 of the row at march index 0
+    int64_t be_step, al_step;  // per march step
+};
+
+template <typename T, int NDIM>
+__device__ __forceinline__ AdjX<T> adj_xslab(const AdjointArgs<T>& A, int shot, int64_t i,
+                                             int64_t j) {
+    AdjX<T> X;
+    X.ext = false;
+    X.m = 0;
+    X.v0 = X.v1 = X.v2 = X.e_lo = X.e_hi = X.w_lo = false;
+    X.a0 = X.a1 = X.a2 = X.bK1 = X.bKh_lo = X.ah_lo = X.bKh_hi = X.ah_hi = T(0);
+    X.be_rd = X.al_rd = nullptr;
+    X.be_wr = X.al_wr = nullptr;
+    X.be0 = X.al0 = X.be_step = X.al_step = 0;
+    const int64_t h = A.halo, n = A.n[0], ny = A.n[1];
+    int side = -1;
+    int64_t m = 0;
+    for (int s = 0; s < 2 && side < 0; ++s) {
+        if (!A.active[0][s]) continue;
+        const int64_t mm = i - (s ? n - h - 2 : 0);
+        if (mm >= 0 && mm <= h + 1) {
+            side = s;
+            m = mm;
+        }
+    }
+    if (side < 0) return X;
+    const AdjSide<T>& S = A.side[0][side];
+    X.ext = true;
+    X.m = (int)m;
+    X.v0 = m - 1 >= 1 && m - 1 <= h;
+    X.v1 = m >= 1 && m <= h;
+    X.v2 = m + 1 >= 1 && m + 1 <= h;
+    if (X.v0) X.a0 = S.a[m - 2];
+    if (X.v1) {
+        X.a1 = S.a[m - 1];
+        X.bK1 = S.bK[m - 1];
+    }
+    if (X.v2) X.a2 = S.a[m];
+    X.e_lo = m >= 1;
+    X.e_hi = m <= h;
+    X.w_lo = m == 1 && side == 0;
+    if (X.e_lo) {
+        X.bKh_lo = S.bKh[m - 1];
+        X.ah_lo = S.ah[m - 1];
+    }
+    if (X.e_hi) {
+        X.bKh_hi = S.bKh[m];
+        X.ah_hi = S.ah[m];
+    }
+    X.be_rd = S.be_rd + shot * A.xi_stride[0];
+    X.be_wr = S.be_wr + shot * A.xi_stride[0];
+    X.al_rd = S.al_rd + shot * A.psi_stride[0];
+    X.al_wr = S.al_wr + shot * A.psi_stride[0];
+    // slab_index(0, L, ...) = L + layers * (j + ny * k): 3D march over k,
+    // 2D march over the second index (passed as j, with k = 0)
+    if (NDIM == 3) {
+        X.be0 = h * j;
+        X.al0 = (h + 1) * j;
+        X.be_step = h * ny;
+        X.al_step = (h + 1) * ny;
+    } else {
+        X.be_step = h;
+        X.al_step = h + 1;
+    }
+    return X;
+}
+
 // One node's adjoint update.  (i, j, k) interior (2D: j is the march index
 // and k = 0), lin its element offset in the shot's field; fc, lc: fact and
 // lam^{s+1} at the node; um/up: f lam^{s+1} one node below/above along each
 // axis; l2, qv, gv: lam^{s+2}, q^s and the gradient at the node.
 template <typename T, int NDIM>
-__device__ __forceinline__ void adjoint_core(const AdjointArgs<T>& A, int shot, int64_t i,
+__device__ __forceinline__ void adjoint_core(const AdjointArgs<T>& A, const AdjX<T>& X, int64_t km,
+                                             int shot, int64_t i,
                                              int64_t j, int64_t k, int64_t lin, int64_t fs,
                                              T fc, T lc, const T* um, const T* up, T l2, T qv,
                                              T gv) {
@@ -96,13 +182,43 @@ __device__ __forceinline__ void adjoint_core(const AdjointArgs<T>& A, int shot, 
     for (int a = 0; a < NDIM; ++a) acc += (up[a] - T(2) * uc + um[a]) * A.inv_d2[a];
     const int64_t h = A.halo;
     const int64_t c[3] = {i, j, k};
-    // slab sides of this node whose beta must be advanced after lam^s
+    // x axis from the lane's precomputed data (km: march index)
+    T xb1 = T(0);
+    int64_t xbo = 0;
+    if (X.ext) {
+        xbo = X.be0 + km * X.be_step;
+        const int64_t xao = X.al0 + km * X.al_step;
+        const int mm = X.m;
+        const T b0 = X.v0 ? X.be_rd[xbo + mm - 2] : T(0);
+        xb1 = X.v1 ? X.be_rd[xbo + mm - 1] : T(0);
+        const T b2 = X.v2 ? X.be_rd[xbo + mm] : T(0);
+        const T z0 = X.v0 ? X.a0 * b0 : T(0);
+        const T z1 = X.v1 ? X.a1 * xb1 : T(0);
+        const T z2 = X.v2 ? X.a2 * b2 : T(0);
+        const T w0 = X.v0 ? um[0] + z0 : T(0);
+        const T w1 = X.v1 ? uc + z1 : T(0);
+        const T w2 = X.v2 ? up[0] + z2 : T(0);
+        T y_lo = T(0), y_hi = T(0);
+        if (X.e_lo) {
+            const T alv = X.bKh_lo * X.al_rd[xao + mm - 1] + (w0 - w1) * A.inv_d[0];
+            y_lo = X.ah_lo * alv;
+            if (X.w_lo) X.al_wr[xao + mm - 1] = alv;
+        }
+        if (X.e_hi) {
+            const T alv = X.bKh_hi * X.al_rd[xao + mm] + (w1 - w2) * A.inv_d[0];
+            y_hi = X.ah_hi * alv;
+            X.al_wr[xao + mm] = alv;
+        }
+        acc += (y_lo - y_hi) * A.inv_d[0];
+        acc += (z0 - T(2) * z1 + z2) * A.inv_d2[0];
+    }
+    // slab sides (y, z) of this node whose beta must be advanced after lam^s
     int bs_axis[3], bs_side[3], bs_layer[3];
     int64_t bs_idx[3];
     T bs_old[3];
     int nbs = 0;
 #pragma unroll
-    for (int a = 0; a < NDIM; ++a) {
+    for (int a = 1; a < NDIM; ++a) {
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             if (!A.active[a][s]) continue;
@@ -154,6 +270,7 @@ __device__ __forceinline__ void adjoint_core(const AdjointArgs<T>& A, int shot, 
         }
     }
     A.lam0[fs + lin] = acc;
+    if (X.ext && X.v1) X.be_wr[xbo + X.m - 1] = X.bK1 * xb1 + fc * acc;
     for (int b = 0; b < nbs; ++b) {
         const int a = bs_axis[b], s = bs_side[b];
         const AdjSide<T>& S = A.side[a][s];
@@ -236,6 +353,7 @@ __global__ __launch_bounds__(256) void adjoint_step(const AdjointArgs<T> A) {
     T lp = l1[lin + ms], fp = f[lin + ms];
     AdjNear<T> nr;
     adj_load_near<T, NDIM>(l1, f, l2, qq, gg, lin, px, nr);
+    const AdjX<T> X = adj_xslab<T, NDIM>(A, shot, i, j);
     for (int64_t k = k0; k < k1; ++k, lin += ms) {
         // next node's loads (planes stay inside the field at the chunk end)
         const int64_t kk = k + 2 < mext ? k + 2 : mext - 1;
@@ -253,8 +371,10 @@ __global__ __launch_bounds__(256) void adjoint_step(const AdjointArgs<T> A) {
         }
         um[NDIM - 1] = u_m;
         up[NDIM - 1] = fp * lp;
-        if (NDIM == 3) adjoint_core<T, 3>(A, shot, i, j, k, lin, fs, fc, lc, um, up, nr.l2, nr.qv, nr.gv);
-        else adjoint_core<T, 2>(A, shot, i, k, 0, lin, fs, fc, lc, um, up, nr.l2, nr.qv, nr.gv);
+        if (NDIM == 3)
+            adjoint_core<T, 3>(A, X, k, shot, i, j, k, lin, fs, fc, lc, um, up, nr.l2, nr.qv, nr.gv);
+        else
+            adjoint_core<T, 2>(A, X, k, shot, i, k, 0, lin, fs, fc, lc, um, up, nr.l2, nr.qv, nr.gv);
         u_m = fc * lc;
         lc = lp;
         fc = fp;
