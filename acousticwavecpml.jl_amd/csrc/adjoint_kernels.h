@@ -80,202 +80,217 @@ __device__ __forceinline__ int64_t slab_index(int a, int64_t layers, int64_t L, 
     return i + px * (j + ny * L);
 }
 
-// x-axis CPML data of one lane (constant along the march): whether the
-// lane's column lies in the range 0 <= m <= halo+1 of an active x side,
-// the coefficients of the layers and edges it touches, and the offsets of
-// its alpha/beta rows.  Only the slab values themselves are loaded per node.
+// CPML data of one node on one axis.  The node lies at m = c - off in the
+// range 0 <= m <= halo+1 of at most one active side; v0/v1/v2 say whether
+// the nodes m-1, m, m+1 are slab layers (1..halo), e_lo/e_hi whether the
+// alpha edges m-1 and m exist, w_lo whether this node stores edge m-1
+// (edge 0 of the low side, which no other node owns).
 template <typename T>
-struct AdjX {
-    bool ext;
-    int m;
-    bool v0, v1, v2;           // mm = m-1, m, m+1 are slab layers (1..halo)
-    T a0, a1, a2;              // a[mm-1] of those layers
-    T bK1;                     // bK[m-1] (the node's own beta)
-    bool e_lo, e_hi, w_lo;     // edges m-1 / m exist; this lane writes alpha[m-1]
-    T bKh_lo, ah_lo, bKh_hi, ah_hi;
+struct AdjCoef {
+    bool ext, v0, v1, v2, e_lo, e_hi, w_lo;
+    int side, m;
+    T a0, a1, a2;      // a of layers m-1, m, m+1
+    T bK1;             // bK of layer m (the node's own beta)
+    T bKh_lo, ah_lo;   // edge m-1
+    T bKh_hi, ah_hi;   // edge m
+};
+
+template <typename T>
+__device__ __forceinline__ AdjCoef<T> adj_coef(const AdjointArgs<T>& A, int a, int64_t c) {
+    AdjCoef<T> C;
+    C.ext = C.v0 = C.v1 = C.v2 = C.e_lo = C.e_hi = C.w_lo = false;
+    C.side = 0;
+    C.m = 0;
+    C.a0 = C.a1 = C.a2 = C.bK1 = C.bKh_lo = C.ah_lo = C.bKh_hi = C.ah_hi = T(0);
+    const int64_t h = A.halo;
+    int64_t m = -1;
+    int side = 0;
+    if (A.active[a][0] && c <= h + 1) {
+        m = c;
+    } else if (A.active[a][1] && c >= A.n[a] - h - 2) {
+        m = c - (A.n[a] - h - 2);
+        side = 1;
+    }
+    if (m < 0 || m > h + 1) return C;
+    const T* av = side ? A.side[a][1].a : A.side[a][0].a;
+    const T* bK = side ? A.side[a][1].bK : A.side[a][0].bK;
+    const T* bKh = side ? A.side[a][1].bKh : A.side[a][0].bKh;
+    const T* ah = side ? A.side[a][1].ah : A.side[a][0].ah;
+    C.ext = true;
+    C.side = side;
+    C.m = (int)m;
+    C.v0 = m >= 2 && m - 1 <= h;
+    C.v1 = m >= 1 && m <= h;
+    C.v2 = m + 1 <= h;
+    if (C.v0) C.a0 = av[m - 2];
+    if (C.v1) {
+        C.a1 = av[m - 1];
+        C.bK1 = bK[m - 1];
+    }
+    if (C.v2) C.a2 = av[m];
+    C.e_lo = m >= 1;
+    C.e_hi = m <= h;
+    C.w_lo = m == 1 && side == 0;
+    if (C.e_lo) {
+        C.bKh_lo = bKh[m - 1];
+        C.ah_lo = ah[m - 1];
+    }
+    if (C.e_hi) {
+        C.bKh_hi = bKh[m];
+        C.ah_hi = ah[m];
+    }
+    return C;
+}
+
+// The slab values one node reads on one axis: beta^{s+1} of layers m-1,
+// m, m+1 and alpha^{s+2} of edges m-1, m.
+template <typename T>
+struct AdjVals {
+    T b0, b1, b2, alo, ahi;
+};
+
+// ob: offset of the node's own beta (layer m); oa: offset of alpha edge
+// m-1; ls: offset between neighbouring layers/edges.
+template <typename T>
+__device__ __forceinline__ void adj_vals(const AdjCoef<T>& C, const T* be, const T* al,
+                                         int64_t ob, int64_t oa, int64_t ls, AdjVals<T>& V) {
+    V.b0 = C.v0 ? be[ob - ls] : T(0);
+    V.b1 = C.v1 ? be[ob] : T(0);
+    V.b2 = C.v2 ? be[ob + ls] : T(0);
+    V.alo = C.e_lo ? al[oa] : T(0);
+    V.ahi = C.e_hi ? al[oa + ls] : T(0);
+}
+
+// One axis's slab terms of lam^s (added to acc) and its alpha^{s+1} stores.
+// um/uc/up: f lam^{s+1} one node below, at and above the node on the axis.
+template <typename T>
+__device__ __forceinline__ void adj_term(const AdjCoef<T>& C, const AdjVals<T>& V, T um, T uc,
+                                         T up, T inv_d, T inv_d2, T* alw, int64_t oa,
+                                         int64_t ls, T& acc) {
+    const T z0 = C.v0 ? C.a0 * V.b0 : T(0);
+    const T z1 = C.v1 ? C.a1 * V.b1 : T(0);
+    const T z2 = C.v2 ? C.a2 * V.b2 : T(0);
+    const T w0 = C.v0 ? um + z0 : T(0);
+    const T w1 = C.v1 ? uc + z1 : T(0);
+    const T w2 = C.v2 ? up + z2 : T(0);
+    T y_lo = T(0), y_hi = T(0);
+    if (C.e_lo) {
+        const T alv = C.bKh_lo * V.alo + (w0 - w1) * inv_d;
+        y_lo = C.ah_lo * alv;
+        if (C.w_lo) alw[oa] = alv;
+    }
+    if (C.e_hi) {
+        const T alv = C.bKh_hi * V.ahi + (w1 - w2) * inv_d;
+        y_hi = C.ah_hi * alv;
+        alw[oa + ls] = alv;
+    }
+    acc += (y_lo - y_hi) * inv_d;
+    acc += (z0 - T(2) * z1 + z2) * inv_d2;
+}
+
+// An axis across the march (x, and y in 3D): its CPML data is the same for
+// every node of a thread's march, and the slab offsets advance by a fixed
+// stride per march step.
+template <typename T>
+struct AdjAxis {
+    AdjCoef<T> C;
     const T* be_rd;
     T* be_wr;
     const T* al_rd;
     T* al_wr;
-    int64_t be0, al0;          // element offset of layer 0 of the row at march index 0
-    int64_t be_step, al_step;  // per march step
+    int64_t ob0, oa0;   // offsets (own beta layer, alpha edge m-1) at march index 0
+    int64_t ls;         // layer stride
+    int64_t sb, sa;     // beta / alpha offset per march step
 };
 
+// Offsets follow slab_index: x slabs are [march][row][layer] (layer
+// fastest), y slabs (3D) are [z][layer][x].
 template <typename T, int NDIM>
-__device__ __forceinline__ AdjX<T> adj_xslab(const AdjointArgs<T>& A, int shot, int64_t i,
-                                             int64_t j) {
-    AdjX<T> X;
-    X.ext = false;
-    X.m = 0;
-    X.v0 = X.v1 = X.v2 = X.e_lo = X.e_hi = X.w_lo = false;
-    X.a0 = X.a1 = X.a2 = X.bK1 = X.bKh_lo = X.ah_lo = X.bKh_hi = X.ah_hi = T(0);
-    X.be_rd = X.al_rd = nullptr;
-    X.be_wr = X.al_wr = nullptr;
-    X.be0 = X.al0 = X.be_step = X.al_step = 0;
-    const int64_t h = A.halo, n = A.n[0], ny = A.n[1];
-    int side = -1;
-    int64_t m = 0;
-    for (int s = 0; s < 2 && side < 0; ++s) {
-        if (!A.active[0][s]) continue;
-        const int64_t mm = i - (s ? n - h - 2 : 0);
-        if (mm >= 0 && mm <= h + 1) {
-            side = s;
-            m = mm;
-        }
-    }
-    if (side < 0) return X;
-    const AdjSide<T>& S = A.side[0][side];
-    X.ext = true;
-    X.m = (int)m;
-    X.v0 = m - 1 >= 1 && m - 1 <= h;
-    X.v1 = m >= 1 && m <= h;
-    X.v2 = m + 1 >= 1 && m + 1 <= h;
-    if (X.v0) X.a0 = S.a[m - 2];
-    if (X.v1) {
-        X.a1 = S.a[m - 1];
-        X.bK1 = S.bK[m - 1];
-    }
-    if (X.v2) X.a2 = S.a[m];
-    X.e_lo = m >= 1;
-    X.e_hi = m <= h;
-    X.w_lo = m == 1 && side == 0;
-    if (X.e_lo) {
-        X.bKh_lo = S.bKh[m - 1];
-        X.ah_lo = S.ah[m - 1];
-    }
-    if (X.e_hi) {
-        X.bKh_hi = S.bKh[m];
-        X.ah_hi = S.ah[m];
-    }
-    X.be_rd = S.be_rd + shot * A.xi_stride[0];
-    X.be_wr = S.be_wr + shot * A.xi_stride[0];
-    X.al_rd = S.al_rd + shot * A.psi_stride[0];
-    X.al_wr = S.al_wr + shot * A.psi_stride[0];
-    // slab_index(0, L, ...) = L + layers * (j + ny * k): 3D march over k,
-    // 2D march over the second index (passed as j, with k = 0)
-    if (NDIM == 3) {
-        X.be0 = h * j;
-        X.al0 = (h + 1) * j;
-        X.be_step = h * ny;
-        X.al_step = (h + 1) * ny;
+__device__ __forceinline__ AdjAxis<T> adj_axis(const AdjointArgs<T>& A, int a, int shot,
+                                               int64_t i, int64_t j) {
+    AdjAxis<T> X;
+    X.C = adj_coef<T>(A, a, a == 0 ? i : j);
+    const int s = X.C.side;
+    X.be_rd = (s ? A.side[a][1].be_rd : A.side[a][0].be_rd) + shot * A.xi_stride[a];
+    X.be_wr = (s ? A.side[a][1].be_wr : A.side[a][0].be_wr) + shot * A.xi_stride[a];
+    X.al_rd = (s ? A.side[a][1].al_rd : A.side[a][0].al_rd) + shot * A.psi_stride[a];
+    X.al_wr = (s ? A.side[a][1].al_wr : A.side[a][0].al_wr) + shot * A.psi_stride[a];
+    const int64_t h = A.halo, ny = A.n[1], px = A.px, L = X.C.m - 1;
+    if (a == 0) {
+        X.ls = 1;
+        X.ob0 = L + (NDIM == 3 ? h * j : 0);
+        X.oa0 = L + (NDIM == 3 ? (h + 1) * j : 0);
+        X.sb = NDIM == 3 ? h * ny : h;
+        X.sa = NDIM == 3 ? (h + 1) * ny : h + 1;
     } else {
-        X.be_step = h;
-        X.al_step = h + 1;
+        X.ls = px;
+        X.ob0 = i + px * L;
+        X.oa0 = X.ob0;
+        X.sb = px * h;
+        X.sa = px * (h + 1);
     }
     return X;
 }
 
+template <typename T>
+__device__ __forceinline__ void adj_axis_vals(const AdjAxis<T>& X, int64_t k, AdjVals<T>& V) {
+    adj_vals(X.C, X.be_rd, X.al_rd, X.ob0 + k * X.sb, X.oa0 + k * X.sa, X.ls, V);
+}
+
+// The march axis: its slab layers are whole planes, layer L of a thread's
+// column at base + L * ms (ms: march stride), as its field nodes.
+template <typename T>
+__device__ __forceinline__ void adj_march_vals(const AdjointArgs<T>& A, const AdjCoef<T>& C,
+                                               int a, int shot, int64_t base, int64_t ms,
+                                               AdjVals<T>& V) {
+    const int s = C.side;
+    const T* be = (s ? A.side[a][1].be_rd : A.side[a][0].be_rd) + shot * A.xi_stride[a];
+    const T* al = (s ? A.side[a][1].al_rd : A.side[a][0].al_rd) + shot * A.psi_stride[a];
+    const int64_t o = base + (int64_t)(C.m - 1) * ms;
+    adj_vals(C, be, al, o, o, ms, V);
+}
+
 // One node's adjoint update.  (i, j, k) interior (2D: j is the march index
-// and k = 0), lin its element offset in the shot's field; fc, lc: fact and
-// lam^{s+1} at the node; um/up: f lam^{s+1} one node below/above along each
-// axis; l2, qv, gv: lam^{s+2}, q^s and the gradient at the node.
+// and k = 0), lin its element offset in the shot's field, km its march
+// index; fc, lc: fact and lam^{s+1} at the node; um/up: f lam^{s+1} one
+// node below/above along each axis; l2, qv, gv: lam^{s+2}, q^s and the
+// gradient at the node; X, Y, Z: CPML data of the x, y (3D) and march
+// axes with their slab values xv, yv, zv.
 template <typename T, int NDIM>
-__device__ __forceinline__ void adjoint_core(const AdjointArgs<T>& A, const AdjX<T>& X, int64_t km,
-                                             int shot, int64_t i,
-                                             int64_t j, int64_t k, int64_t lin, int64_t fs,
+__device__ __forceinline__ void adjoint_core(const AdjointArgs<T>& A, const AdjAxis<T>& X,
+                                             const AdjAxis<T>& Y, const AdjCoef<T>& Z,
+                                             const AdjVals<T>& xv, const AdjVals<T>& yv,
+                                             const AdjVals<T>& zv, int64_t km, int shot,
+                                             int64_t base, int64_t ms, int64_t lin, int64_t fs,
                                              T fc, T lc, const T* um, const T* up, T l2, T qv,
                                              T gv) {
-    const int64_t px = A.px, ny = A.n[1];
     const T uc = fc * lc;
     T acc = T(2) * lc - l2;
 #pragma unroll
     for (int a = 0; a < NDIM; ++a) acc += (up[a] - T(2) * uc + um[a]) * A.inv_d2[a];
-    const int64_t h = A.halo;
-    const int64_t c[3] = {i, j, k};
-    // x axis from the lane's precomputed data (km: march index)
-    T xb1 = T(0);
-    int64_t xbo = 0;
-    if (X.ext) {
-        xbo = X.be0 + km * X.be_step;
-        const int64_t xao = X.al0 + km * X.al_step;
-        const int mm = X.m;
-        const T b0 = X.v0 ? X.be_rd[xbo + mm - 2] : T(0);
-        xb1 = X.v1 ? X.be_rd[xbo + mm - 1] : T(0);
-        const T b2 = X.v2 ? X.be_rd[xbo + mm] : T(0);
-        const T z0 = X.v0 ? X.a0 * b0 : T(0);
-        const T z1 = X.v1 ? X.a1 * xb1 : T(0);
-        const T z2 = X.v2 ? X.a2 * b2 : T(0);
-        const T w0 = X.v0 ? um[0] + z0 : T(0);
-        const T w1 = X.v1 ? uc + z1 : T(0);
-        const T w2 = X.v2 ? up[0] + z2 : T(0);
-        T y_lo = T(0), y_hi = T(0);
-        if (X.e_lo) {
-            const T alv = X.bKh_lo * X.al_rd[xao + mm - 1] + (w0 - w1) * A.inv_d[0];
-            y_lo = X.ah_lo * alv;
-            if (X.w_lo) X.al_wr[xao + mm - 1] = alv;
-        }
-        if (X.e_hi) {
-            const T alv = X.bKh_hi * X.al_rd[xao + mm] + (w1 - w2) * A.inv_d[0];
-            y_hi = X.ah_hi * alv;
-            X.al_wr[xao + mm] = alv;
-        }
-        acc += (y_lo - y_hi) * A.inv_d[0];
-        acc += (z0 - T(2) * z1 + z2) * A.inv_d2[0];
+    const int M = NDIM - 1;
+    const int64_t xo = X.oa0 + km * X.sa;
+    if (X.C.ext) adj_term(X.C, xv, um[0], uc, up[0], A.inv_d[0], A.inv_d2[0], X.al_wr, xo, X.ls, acc);
+    int64_t yo = 0;
+    if (NDIM == 3) {
+        yo = Y.oa0 + km * Y.sa;
+        if (Y.C.ext)
+            adj_term(Y.C, yv, um[1], uc, up[1], A.inv_d[1], A.inv_d2[1], Y.al_wr, yo, Y.ls, acc);
     }
-    // slab sides (y, z) of this node whose beta must be advanced after lam^s
-    int bs_axis[3], bs_side[3], bs_layer[3];
-    int64_t bs_idx[3];
-    T bs_old[3];
-    int nbs = 0;
-#pragma unroll
-    for (int a = 1; a < NDIM; ++a) {
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            if (!A.active[a][s]) continue;
-            const int64_t off = s ? A.n[a] - h - 2 : 0;
-            const int64_t m = c[a] - off;
-            if (m < 0 || m > h + 1) continue;
-            const AdjSide<T>& S = A.side[a][s];
-            const T* be = S.be_rd + shot * A.xi_stride[a];
-            const T* al = S.al_rd + shot * A.psi_stride[a];
-            T* alw = S.al_wr + shot * A.psi_stride[a];
-            T z[3], w[3];
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                const int64_t mm = m + d - 1;
-                const T u = d == 0 ? um[a] : (d == 1 ? uc : up[a]);
-                if (mm >= 1 && mm <= h) {
-                    const T bv = be[slab_index(a, h, mm - 1, i, j, k, px, ny)];
-                    z[d] = S.a[mm - 1] * bv;
-                    w[d] = u + z[d];
-                    if (d == 1) {
-                        bs_axis[nbs] = a;
-                        bs_side[nbs] = s;
-                        bs_layer[nbs] = (int)(mm - 1);
-                        bs_idx[nbs] = slab_index(a, h, mm - 1, i, j, k, px, ny);
-                        bs_old[nbs] = bv;
-                        ++nbs;
-                    }
-                } else {
-                    z[d] = T(0);
-                    w[d] = T(0);
-                }
-            }
-            T y_lo = T(0), y_hi = T(0);
-            if (m >= 1) {
-                const int64_t e = m - 1;
-                const int64_t ie = slab_index(a, h + 1, e, i, j, k, px, ny);
-                const T alv = S.bKh[e] * al[ie] + (w[0] - w[1]) * A.inv_d[a];
-                y_lo = S.ah[e] * alv;
-                if (m == 1 && s == 0) alw[ie] = alv;
-            }
-            if (m <= h) {
-                const int64_t ie = slab_index(a, h + 1, m, i, j, k, px, ny);
-                const T alv = S.bKh[m] * al[ie] + (w[1] - w[2]) * A.inv_d[a];
-                y_hi = S.ah[m] * alv;
-                alw[ie] = alv;
-            }
-            acc += (y_lo - y_hi) * A.inv_d[a];
-            acc += (z[0] - T(2) * z[1] + z[2]) * A.inv_d2[a];
-        }
+    const int64_t zo = base + (int64_t)(Z.m - 1) * ms;
+    T* zal = nullptr;
+    T* zbe = nullptr;
+    if (Z.ext) {
+        const int s = Z.side;
+        zal = (s ? A.side[M][1].al_wr : A.side[M][0].al_wr) + shot * A.psi_stride[M];
+        zbe = (s ? A.side[M][1].be_wr : A.side[M][0].be_wr) + shot * A.xi_stride[M];
+        adj_term(Z, zv, um[M], uc, up[M], A.inv_d[M], A.inv_d2[M], zal, zo, ms, acc);
     }
     A.lam0[fs + lin] = acc;
-    if (X.ext && X.v1) X.be_wr[xbo + X.m - 1] = X.bK1 * xb1 + fc * acc;
-    for (int b = 0; b < nbs; ++b) {
-        const int a = bs_axis[b], s = bs_side[b];
-        const AdjSide<T>& S = A.side[a][s];
-        S.be_wr[shot * A.xi_stride[a] + bs_idx[b]] = S.bK[bs_layer[b]] * bs_old[b] + fc * acc;
-    }
+    // beta^s = bK beta^{s+1} + f lam^s on every slab holding the node
+    if (X.C.ext && X.C.v1) X.be_wr[X.ob0 + km * X.sb] = X.C.bK1 * xv.b1 + fc * acc;
+    if (NDIM == 3 && Y.C.ext && Y.C.v1) Y.be_wr[Y.ob0 + km * Y.sb] = Y.C.bK1 * yv.b1 + fc * acc;
+    if (Z.ext && Z.v1) zbe[zo] = Z.bK1 * zv.b1 + fc * acc;
     A.grad[fs + lin] = gv + acc * qv;
 }
 
@@ -308,8 +323,9 @@ __device__ __forceinline__ void adj_load_near(const T* l1, const T* f, const T* 
 // threadIdx.y -> y row (3D) or march chunk (2D); each thread marches a
 // chunk of march_len nodes along the last axis.  f lam^{s+1} along the
 // march is carried in registers (one new plane per node), and every load
-// of node k+1 is issued before node k is computed.  Logical blocks: x tile
-// fastest, then row/chunk group, chunk (3D), shot, renumbered XCD-aware.
+// of node k+1, its CPML slab values included, is issued before node k is
+// computed.  Logical blocks: x tile fastest, then row/chunk group, chunk
+// (3D), shot, renumbered XCD-aware.
 template <typename T, int NDIM>
 __global__ __launch_bounds__(256) void adjoint_step(const AdjointArgs<T> A) {
     const int lb = logical_block(blockIdx.x, A.grid_blocks);
@@ -339,6 +355,7 @@ __global__ __launch_bounds__(256) void adjoint_step(const AdjointArgs<T> A) {
     const int64_t k0 = 1 + (int64_t)chunk * A.march_len;
     const int64_t k1 = k0 + A.march_len < mext - 1 ? k0 + A.march_len : mext - 1;
     if (k0 >= k1) return;
+    const int M = NDIM - 1;
     const int64_t ms = NDIM == 3 ? px * ny : px;  // march stride
     const int64_t fs = (int64_t)shot * A.field_stride;
     const T* l1 = A.lam1 + fs;
@@ -353,13 +370,26 @@ __global__ __launch_bounds__(256) void adjoint_step(const AdjointArgs<T> A) {
     T lp = l1[lin + ms], fp = f[lin + ms];
     AdjNear<T> nr;
     adj_load_near<T, NDIM>(l1, f, l2, qq, gg, lin, px, nr);
-    const AdjX<T> X = adj_xslab<T, NDIM>(A, shot, i, j);
+    const AdjAxis<T> X = adj_axis<T, NDIM>(A, 0, shot, i, j);
+    AdjAxis<T> Y;
+    if (NDIM == 3) Y = adj_axis<T, NDIM>(A, 1, shot, i, j);
+    else Y.C.ext = false;
+    AdjCoef<T> Z = adj_coef<T>(A, M, k0);
+    AdjVals<T> xv = {}, yv = {}, zv = {};
+    if (X.C.ext) adj_axis_vals(X, k0, xv);
+    if (NDIM == 3 && Y.C.ext) adj_axis_vals(Y, k0, yv);
+    if (Z.ext) adj_march_vals(A, Z, M, shot, base, ms, zv);
     for (int64_t k = k0; k < k1; ++k, lin += ms) {
         // next node's loads (planes stay inside the field at the chunk end)
         const int64_t kk = k + 2 < mext ? k + 2 : mext - 1;
         const T lpp = l1[base + kk * ms], fpp = f[base + kk * ms];
         AdjNear<T> nn;
         adj_load_near<T, NDIM>(l1, f, l2, qq, gg, lin + ms, px, nn);
+        const AdjCoef<T> Zn = adj_coef<T>(A, M, k + 1);
+        AdjVals<T> xn = {}, yn = {}, zn = {};
+        if (X.C.ext) adj_axis_vals(X, k + 1, xn);
+        if (NDIM == 3 && Y.C.ext) adj_axis_vals(Y, k + 1, yn);
+        if (Zn.ext) adj_march_vals(A, Zn, M, shot, base, ms, zn);
         // keep the next node's loads ahead of this node's arithmetic
         __builtin_amdgcn_sched_barrier(0);
         T um[3], up[3];
@@ -369,18 +399,20 @@ __global__ __launch_bounds__(256) void adjoint_step(const AdjointArgs<T> A) {
             um[1] = nr.fym * nr.lym;
             up[1] = nr.fyp * nr.lyp;
         }
-        um[NDIM - 1] = u_m;
-        up[NDIM - 1] = fp * lp;
-        if (NDIM == 3)
-            adjoint_core<T, 3>(A, X, k, shot, i, j, k, lin, fs, fc, lc, um, up, nr.l2, nr.qv, nr.gv);
-        else
-            adjoint_core<T, 2>(A, X, k, shot, i, k, 0, lin, fs, fc, lc, um, up, nr.l2, nr.qv, nr.gv);
+        um[M] = u_m;
+        up[M] = fp * lp;
+        adjoint_core<T, NDIM>(A, X, Y, Z, xv, yv, zv, k, shot, base, ms, lin, fs, fc, lc, um, up,
+                              nr.l2, nr.qv, nr.gv);
         u_m = fc * lc;
         lc = lp;
         fc = fp;
         lp = lpp;
         fp = fpp;
         nr = nn;
+        Z = Zn;
+        xv = xn;
+        yv = yn;
+        zv = zn;
     }
 }
 
