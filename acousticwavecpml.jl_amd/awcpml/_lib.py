@@ -50,6 +50,10 @@ SIGNATURES = {
                                          ctypes.POINTER(ctypes.c_int)]),
     "awcpml_set_shot_group": (ctypes.c_int, [_vp, ctypes.c_int]),
     "awcpml_get_shot_group": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
+    "awcpml_set_time_block": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int]),
+    "awcpml_get_time_block": (ctypes.c_int, [_vp] + [ctypes.POINTER(ctypes.c_int)] * 4),
+    "awcpml_time_stencil2": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "awcpml_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
     "awcpml_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(_vp)]),
     "awcpml_batch_destroy": (ctypes.c_int, [_vp]),

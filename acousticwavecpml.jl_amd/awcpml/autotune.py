@@ -10,7 +10,9 @@ reset afterwards.
     best = autotune(plan)            # {"block_y": 4, "march_len": 148, ...}
 
 Multi-shot plans also try shot groups (the waves of a block march the same
-rows of several shots, sharing the model rows they read).
+rows of several shots, sharing the model rows they read).  3D plans also
+time two-step launches (``Plan.set_time_block``: two time steps per launch,
+bit-identical results) per time step, and keep them when they are faster.
 """
 import json
 import os
@@ -36,6 +38,26 @@ def candidates(plan):
     return out
 
 
+def two_step_candidates(plan):
+    """(block_rows, march_len, prefetch) of two-step launches worth timing."""
+    if plan.ndim != 3:
+        return []
+    n_march = plan.n[2] - 2
+    rows = (16, 8) if plan.dtype != 0 else (8, 4)
+    lens = sorted({min(n_march, v) for v in (24, 48, 96)})
+    return [(w, ml, pf) for w in rows for ml in lens for pf in (1, 2)]
+
+
+def _apply(plan, best):
+    plan.set_tiling(best["block_y"], best["march_len"], best["prefetch"])
+    plan.set_shot_group(best.get("shot_group", 1))
+    tb = best.get("time_block")
+    if tb:
+        plan.set_time_block(2, tb["block_rows"], tb["march_len"], tb["prefetch"])
+    elif plan.ndim == 3:
+        plan.set_time_block(1)
+
+
 def _key(plan):
     return "%dD:%s:h%d:%s:s%d" % (plan.ndim, "x".join(map(str, plan.n)), plan.halo,
                                    "f64" if plan.dtype == 0 else "f32", plan.nshots)
@@ -55,9 +77,10 @@ def autotune(plan, cands=None, launches=8, cache=DEFAULT_CACHE, use_cache=True):
     db = _load(cache) if cache else {}
     if use_cache and key in db:
         best = db[key]
-        plan.set_tiling(best["block_y"], best["march_len"], best["prefetch"])
-        plan.set_shot_group(best.get("shot_group", 1))
+        _apply(plan, best)
         return dict(best, cached=True)
+    if plan.ndim == 3:
+        plan.set_time_block(1)
     results = []
     for cand in (cands or candidates(plan)):
         by, ml, pf = cand[:3]
@@ -73,7 +96,26 @@ def autotune(plan, cands=None, launches=8, cache=DEFAULT_CACHE, use_cache=True):
     plan.set_tiling(by, ml, pf)
     plan.set_shot_group(g)
     best = {"block_y": by, "march_len": ml, "prefetch": pf, "shot_group": g,
-            "ms_per_launch": ms, "tried": len(results)}
+            "ms_per_launch": ms, "ms_per_step": ms, "time_block": None, "tried": len(results)}
+    # two time steps per launch, compared per time step
+    from . import _lib
+    two = []
+    for w, ml2, pf2 in (two_step_candidates(plan) if cands is None else []):
+        try:
+            plan.set_time_block(2, w, ml2, pf2)
+            plan.reset()
+            plan.time_stencil2(1)  # warm-up
+            n2 = max(1, launches // 2)
+            two.append((plan.time_stencil2(n2) / n2 / 2.0, w, ml2, pf2))
+        except _lib.AwcpmlError:
+            break  # two-step launches do not apply to this plan
+    if two and min(two)[0] < best["ms_per_step"]:
+        ms2, w, ml2, pf2 = min(two)
+        best["time_block"] = {"block_rows": w, "march_len": ml2, "prefetch": pf2}
+        best["ms_per_step"] = ms2
+    best["tried"] += len(two)
+    _apply(plan, best)
+    plan.reset()
     if cache:
         db[key] = best
         try:

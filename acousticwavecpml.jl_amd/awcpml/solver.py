@@ -120,6 +120,18 @@ class Plan:
         _lib.check(_lib.load().awcpml_get_shot_group(self._h, ctypes.byref(g)))
         return g.value
 
+    def set_time_block(self, steps, block_rows=0, march_len=0, prefetch=0):
+        """steps = 2: two time steps per stencil launch where that applies
+        (3D plans); results are bit-identical to one step per launch."""
+        _lib.check(_lib.load().awcpml_set_time_block(self._h, int(steps), int(block_rows),
+                                                     int(march_len), int(prefetch)))
+
+    def time_block(self):
+        """(steps a launch advances now, block_rows, march_len, prefetch)."""
+        v = [ctypes.c_int() for _ in range(4)]
+        _lib.check(_lib.load().awcpml_get_time_block(self._h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
     # -- execution ----------------------------------------------------
     def reset(self):
         _lib.check(_lib.load().awcpml_reset(self._h))
@@ -176,6 +188,12 @@ class Plan:
     def time_stencil(self, nlaunches):
         ms = ctypes.c_double()
         _lib.check(_lib.load().awcpml_time_stencil(self._h, int(nlaunches), ctypes.byref(ms)))
+        return ms.value
+
+    def time_stencil2(self, nlaunches):
+        """Milliseconds of `nlaunches` two-step stencil launches."""
+        ms = ctypes.c_double()
+        _lib.check(_lib.load().awcpml_time_stencil2(self._h, int(nlaunches), ctypes.byref(ms)))
         return ms.value
 
     def time_advance(self, nsteps):

@@ -55,9 +55,35 @@ int fail(int code, const char* fmt, ...) {
 
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
+// Which physical buffer plays which role: the field rotation (p[rot] =
+// pold ...), the psi parity, and the spare buffers of two-step launches,
+// which trade places with the buffers they replace.  A captured graph
+// holds physical addresses, so it is replayed only from the layout it was
+// captured in.
+struct BufLayout {
+    void* p[4];           // p[0..2], spare
+    void* psi[3][2][3];   // psi[a][s][0..1], spare
+    void* xi[3][2][2];    // xi[a][s], spare
+    int rot, par;
+    bool operator==(const BufLayout& o) const {
+        if (rot != o.rot || par != o.par) return false;
+        for (int b = 0; b < 4; ++b)
+            if (p[b] != o.p[b]) return false;
+        for (int a = 0; a < 3; ++a)
+            for (int s = 0; s < 2; ++s) {
+                for (int c = 0; c < 3; ++c)
+                    if (psi[a][s][c] != o.psi[a][s][c]) return false;
+                for (int c = 0; c < 2; ++c)
+                    if (xi[a][s][c] != o.xi[a][s][c]) return false;
+            }
+        return true;
+    }
+};
+
 struct GraphEntry {
     int64_t it0, n;
     hipGraphExec_t exec;
+    BufLayout before, after;  // plans only
 };
 
 void drop_graphs(std::vector<GraphEntry>& g) {
@@ -124,6 +150,18 @@ struct awcpml_plan {
     bool ck_valid = false;
     std::vector<int> ck_rot, ck_par;
     void* q_cur = nullptr;  // d(pnew)/d(fact) output of the next stencil launch
+
+    // two time steps per launch (3D): step t+2's field, psi and xi go to
+    // spare buffers, which then trade places with the buffers they replace
+    int time_block = 1;
+    int tb_rows = 0, tb_march = 0;   // waves per block, chunk length (0: default)
+    int tb_prefetch = 0;             // planes of loads in flight (0: default)
+    void* p_spare = nullptr;
+    void* psi_spare[3][2] = {};
+    void* xi_spare[3][2] = {};
+    BufLayout home = {};             // the layout a reset restores
+    bool src_on_boundary = false;    // some source lies on a Dirichlet node
+    bool boundary_dirty = false;     // such a source was injected since the reset
 };
 
 namespace {
@@ -133,6 +171,38 @@ int64_t slab_elems(const awcpml_plan* P, int axis, int64_t layers) {
     if (axis == 0) return layers * P->n[1] * P->n[2];
     if (axis == 1) return P->px * layers * P->n[2];
     return P->px * P->n[1] * layers;
+}
+
+BufLayout get_layout(const awcpml_plan* P) {
+    BufLayout L = {};
+    for (int b = 0; b < 3; ++b) L.p[b] = P->p[b];
+    L.p[3] = P->p_spare;
+    for (int a = 0; a < 3; ++a)
+        for (int s = 0; s < 2; ++s) {
+            L.psi[a][s][0] = P->psi[a][s][0];
+            L.psi[a][s][1] = P->psi[a][s][1];
+            L.psi[a][s][2] = P->psi_spare[a][s];
+            L.xi[a][s][0] = P->xi[a][s];
+            L.xi[a][s][1] = P->xi_spare[a][s];
+        }
+    L.rot = P->rot;
+    L.par = P->par;
+    return L;
+}
+
+void set_layout(awcpml_plan* P, const BufLayout& L) {
+    for (int b = 0; b < 3; ++b) P->p[b] = L.p[b];
+    P->p_spare = L.p[3];
+    for (int a = 0; a < 3; ++a)
+        for (int s = 0; s < 2; ++s) {
+            P->psi[a][s][0] = L.psi[a][s][0];
+            P->psi[a][s][1] = L.psi[a][s][1];
+            P->psi_spare[a][s] = L.psi[a][s][2];
+            P->xi[a][s] = L.xi[a][s][0];
+            P->xi_spare[a][s] = L.xi[a][s][1];
+        }
+    P->rot = L.rot;
+    P->par = L.par;
 }
 
 int dalloc(void** ptr, size_t bytes) {
@@ -283,6 +353,7 @@ awcpml::IoArgs<T> make_io(const awcpml_plan* P, int rot) {
     A.traces = (T*)P->d_traces;
     A.rec_start = (const int*)P->d_rec_start;
     A.nt = P->nt;
+    A.inject = 1;
     return A;
 }
 
@@ -348,6 +419,13 @@ int sync_io(awcpml_plan* P) {
     HIP_TRY(hipStreamSynchronize(P->stream));
     P->n_src_total = (int64_t)slin.size();
     P->n_rec_total = (int64_t)rlin.size();
+    P->src_on_boundary = false;
+    for (int64_t lin : slin) {
+        const int64_t i = lin % P->px, j = (lin / P->px) % P->n[1], k = lin / (P->px * P->n[1]);
+        bool b = i == 0 || i == P->n[0] - 1 || j == 0 || j == P->n[1] - 1;
+        if (P->ndim == 3) b = b || k == 0 || k == P->n[2] - 1;
+        P->src_on_boundary = P->src_on_boundary || b;
+    }
     P->has_io = (P->n_src_total + P->n_rec_total) > 0;
     P->io_dirty = false;
     return AWCPML_OK;
@@ -368,8 +446,140 @@ int do_step(awcpml_plan* P, int64_t it0) {
     int st = launch_stencil(P);
     if (st) return st;
     if (P->has_io && (st = launch_io(P, it0))) return st;
+    if (P->has_io && P->src_on_boundary) P->boundary_dirty = true;
     P->rot = (P->rot + 1) % 3;
     P->par ^= 1;
+    return AWCPML_OK;
+}
+
+// Two-step launches apply: requested, 3D, every field buffer zero on the
+// Dirichlet nodes, 32-bit byte offsets, no d(pnew)/d(fact) output.
+bool two_ok(const awcpml_plan* P) {
+    return P->time_block == 2 && P->ndim == 3 && P->p_spare && !P->q_cur && !P->src_on_boundary &&
+           !P->boundary_dirty && (uint64_t)P->field_elems * P->esize < (1ull << 32);
+}
+
+void two_tiling(const awcpml_plan* P, int* rows, int* march, int* prefetch) {
+    int W = P->tb_rows;
+    if (W <= 0) W = P->dtype == AWCPML_F64 ? 8 : 16;
+    int ml = P->tb_march;
+    if (ml <= 0) {
+        // enough blocks to fill 256 CUs several times over
+        const int64_t tiles = ((P->n[0] - 2 + awcpml::kOwnX - 1) / awcpml::kOwnX) *
+                              ((P->n[1] - 2 + W - 3) / (W - 2)) * P->nshots;
+        const int64_t interior = P->n[2] - 2;
+        const int64_t want = std::max<int64_t>(1, (2048 + tiles - 1) / tiles);
+        ml = (int)std::max<int64_t>(16, std::min<int64_t>((interior + want - 1) / want, 128));
+    }
+    *rows = W;
+    *march = ml;
+    *prefetch = P->tb_prefetch > 0 ? P->tb_prefetch : 2;
+}
+
+// Arguments of a two-step launch from the current layout: step t+1 reads
+// p[rot] (pold), p[rot+1] (pcur), psi[par] and xi and writes p[rot+2];
+// step t+2 writes the spare field, psi and xi buffers.
+template <typename T>
+awcpml::Stencil2Args<T> make_args2(const awcpml_plan* P, int64_t it0, int W, int ml) {
+    awcpml::Stencil2Args<T> B;
+    memset(&B, 0, sizeof B);
+    B.s = make_args<T>(P, P->rot, P->par);
+    StencilArgs<T>& A = B.s;
+    for (int a = 0; a < 3; ++a)
+        for (int s = 0; s < 2; ++s) {
+            A.side[a][s].psi_wr = (T*)P->psi_spare[a][s];
+            B.xi_out[a][s] = (T*)P->xi_spare[a][s];
+        }
+    A.qout = nullptr;
+    A.shot_group = 1;
+    A.nsgroups = P->nshots;
+    A.tiles_x = (int)((P->n[0] - 2 + awcpml::kOwnX - 1) / awcpml::kOwnX);
+    A.tiles_y = (int)((P->n[1] - 2 + W - 3) / (W - 2));
+    A.march_len = ml;
+    A.nchunks = (int)((P->n[2] - 2 + ml - 1) / ml);
+    A.logical_blocks = A.tiles_x * A.tiles_y * A.nchunks * P->nshots;
+    A.grid_blocks = (int)round_up(A.logical_blocks, awcpml::kNumXcd);
+    B.p1 = (T*)P->p[(P->rot + 2) % 3];
+    B.p2 = (T*)P->p_spare;
+    B.src_lin = (const int64_t*)P->d_src_lin;
+    B.src_wav = (const T*)P->d_src_wav;
+    B.src_start = (const int*)P->d_src_start;
+    B.nt = P->nt;
+    B.it0 = it0;
+    return B;
+}
+
+template <typename T>
+int launch_stencil2_t(awcpml_plan* P, int64_t it0) {
+    int W, ml, pf;
+    two_tiling(P, &W, &ml, &pf);
+    const awcpml::Stencil2Args<T> B = make_args2<T>(P, it0, W, ml);
+    awcpml::LaunchShape L;
+    L.grid = dim3(B.s.grid_blocks);
+    L.block = dim3(awcpml::kTileX, W);
+    L.lds = (size_t)8 * (P->halo + 1) * sizeof(T);
+    L.stream = P->stream;
+    awcpml::launch_plan_stencil2<T>(B, L, W, pf);
+    HIP_TRY(hipGetLastError());
+    return AWCPML_OK;
+}
+
+// After a two-step launch: the spares hold step t+2 and take the places of
+// p[rot] (the reference's pnew of step t+2), psi[par] and xi.
+void two_swap(awcpml_plan* P) {
+    std::swap(P->p[P->rot % 3], P->p_spare);
+    for (int a = 0; a < 3; ++a)
+        for (int s = 0; s < 2; ++s) {
+            std::swap(P->psi[a][s][P->par], P->psi_spare[a][s]);
+            std::swap(P->xi[a][s], P->xi_spare[a][s]);
+        }
+    P->rot = (P->rot + 2) % 3;
+}
+
+template <typename T>
+int io_two_t(awcpml_plan* P, int64_t it0, int rot, bool inject) {
+    awcpml::IoArgs<T> A = make_io<T>(P, rot);
+    A.inject = inject ? 1 : 0;
+    awcpml::sources_receivers<T><<<dim3(P->nshots), dim3(256), 0, P->stream>>>(A, it0);
+    HIP_TRY(hipGetLastError());
+    return AWCPML_OK;
+}
+
+// Steps it0 and it0+1 in one stencil launch: the kernel injects step
+// it0's sources; the receivers of step it0 are then read from p^{t+1},
+// and step it0+1's sources and receivers handled on p^{t+2} as usual.
+int do_step2(awcpml_plan* P, int64_t it0) {
+    const bool f64 = P->dtype == AWCPML_F64;
+    int st = f64 ? launch_stencil2_t<double>(P, it0) : launch_stencil2_t<float>(P, it0);
+    if (st) return st;
+    if (P->has_io) {
+        st = f64 ? io_two_t<double>(P, it0, P->rot, false) : io_two_t<float>(P, it0, P->rot, false);
+        if (st) return st;
+    }
+    const int rot1 = (P->rot + 1) % 3;
+    two_swap(P);
+    if (P->has_io) {
+        st = f64 ? io_two_t<double>(P, it0 + 1, rot1, true) : io_two_t<float>(P, it0 + 1, rot1, true);
+        if (st) return st;
+    }
+    return AWCPML_OK;
+}
+
+// Enqueue the steps it0 .. it0+n-1 (pairs in two-step launches where they
+// apply), advancing the host's layout state as they are enqueued.
+int enqueue_steps(awcpml_plan* P, int64_t it0, int64_t n) {
+    const bool two = two_ok(P);
+    for (int64_t q = 0; q < n;) {
+        int st;
+        if (two && n - q >= 2) {
+            st = do_step2(P, it0 + q);
+            q += 2;
+        } else {
+            st = do_step(P, it0 + q);
+            q += 1;
+        }
+        if (st) return st;
+    }
     return AWCPML_OK;
 }
 
@@ -393,6 +603,12 @@ int capture_steps(hipStream_t stream, int64_t n, StepFn step, hipGraphExec_t* ou
     return AWCPML_OK;
 }
 
+// Capture whatever fn() enqueues on `stream` into an executable graph.
+template <typename Fn>
+int capture_seq(hipStream_t stream, Fn fn, hipGraphExec_t* out) {
+    return capture_steps(stream, 1, [&](int64_t) { return fn(); }, out);
+}
+
 GraphEntry* find_graph(std::vector<GraphEntry>& g, int64_t it0, int64_t n) {
     for (auto& e : g)
         if (e.it0 == it0 && e.n == n) return &e;
@@ -403,37 +619,61 @@ GraphEntry* find_graph(std::vector<GraphEntry>& g, int64_t it0, int64_t n) {
     return nullptr;
 }
 
+// The plan's graph for n steps from step it0 and layout L, captured on
+// first use.  The host state (layout, boundary flag) after the capture is
+// the state after the sequence; it is recorded and the plan restored to L.
+int plan_graph(awcpml_plan* P, int64_t n, GraphEntry** out) {
+    const BufLayout now = get_layout(P);
+    const int64_t it0 = P->it;
+    auto& G = P->graphs;
+    for (size_t q = 0; q < G.size(); ++q)
+        if (G[q].it0 == it0 && G[q].n == n) {
+            if (G[q].before == now) {
+                *out = &G[q];
+                return AWCPML_OK;
+            }
+            (void)hipGraphExecDestroy(G[q].exec);
+            G.erase(G.begin() + q);
+            break;
+        }
+    if (G.size() >= 4) {
+        (void)hipGraphExecDestroy(G.front().exec);
+        G.erase(G.begin());
+    }
+    hipGraphExec_t exec;
+    int st = capture_seq(P->stream, [&]() { return enqueue_steps(P, it0, n); }, &exec);
+    const BufLayout after = get_layout(P);
+    set_layout(P, now);
+    if (st) return st;
+    GraphEntry e = {};
+    e.it0 = it0;
+    e.n = n;
+    e.exec = exec;
+    e.before = now;
+    e.after = after;
+    G.push_back(e);
+    *out = &G.back();
+    return AWCPML_OK;
+}
+
 // Enqueue n steps from the plan's current state (no synchronisation).  With
-// graphs enabled the sequence is captured once per (first step, length)
-// and replayed afterwards; the rotation state then follows from the step
-// counter (it never does after awcpml_forward, which falls back to plain
-// launches).
+// graphs enabled the sequence is captured once per (first step, length,
+// buffer layout) and replayed afterwards.
 int run_steps(awcpml_plan* P, int64_t n) {
     if (n <= 0) return AWCPML_OK;
-    if (P->use_graph && P->rot == (int)(P->it % 3) && P->par == (int)(P->it % 2)) {
-        GraphEntry* g = find_graph(P->graphs, P->it, n);
-        if (!g) {
-            const int rot0 = P->rot, par0 = P->par;
-            const int64_t it0 = P->it;
-            hipGraphExec_t exec;
-            int st = capture_steps(P->stream, n, [&](int64_t q) { return do_step(P, it0 + q); }, &exec);
-            P->rot = rot0;
-            P->par = par0;
-            if (st) return st;
-            P->graphs.push_back({it0, n, exec});
-            g = &P->graphs.back();
-        }
+    if (P->use_graph) {
+        GraphEntry* g = nullptr;
+        int st = plan_graph(P, n, &g);
+        if (st) return st;
         HIP_TRY(hipGraphLaunch(g->exec, P->stream));
         P->it += n;
-        P->rot = (int)((P->rot + n) % 3);
-        P->par = (int)((P->par + n) % 2);
+        set_layout(P, g->after);
+        if (P->has_io && P->src_on_boundary) P->boundary_dirty = true;
         return AWCPML_OK;
     }
-    for (int64_t q = 0; q < n; ++q) {
-        int st = do_step(P, P->it);
-        if (st) return st;
-        ++P->it;
-    }
+    int st = enqueue_steps(P, P->it, n);
+    if (st) return st;
+    P->it += n;
     return AWCPML_OK;
 }
 
@@ -643,6 +883,7 @@ int awcpml_plan_create(int ndim, int64_t nx, int64_t ny, int64_t nz, int halo, i
         hipError_t e = hipMemsetAsync(P->fact, 0, (size_t)P->field_elems * P->esize, P->stream);
         if (e != hipSuccess) st = fail(AWCPML_ERR_HIP, "hipMemset failed: %s", hipGetErrorString(e));
     }
+    P->home = get_layout(P);
     if (!st) st = awcpml_reset(P);
     if (!st) st = default_tiling(P);
     if (st) {
@@ -660,12 +901,15 @@ int awcpml_plan_destroy(awcpml_plan* P) {
     if (P->stream) (void)hipStreamSynchronize(P->stream);
     drop_graphs(P->graphs);
     for (int b = 0; b < 3; ++b) dfree(P->p[b]);
+    dfree(P->p_spare);
     dfree(P->fact);
     for (int a = 0; a < 3; ++a) {
         for (int s = 0; s < 2; ++s) {
             dfree(P->psi[a][s][0]);
             dfree(P->psi[a][s][1]);
+            dfree(P->psi_spare[a][s]);
             dfree(P->xi[a][s]);
+            dfree(P->xi_spare[a][s]);
         }
         for (int c = 0; c < 8; ++c) dfree(P->coef[a][c]);
     }
@@ -780,15 +1024,21 @@ int awcpml_set_receivers(awcpml_plan* P, int shot, int nrecs, const int64_t* pos
 int awcpml_reset(awcpml_plan* P) {
     int st = check_plan(P);
     if (st) return st;
+    set_layout(P, P->home);
     const size_t fbytes = (size_t)P->field_elems * P->nshots * P->esize;
     for (int b = 0; b < 3; ++b) HIP_TRY(hipMemsetAsync(P->p[b], 0, fbytes, P->stream));
+    if (P->p_spare) HIP_TRY(hipMemsetAsync(P->p_spare, 0, fbytes, P->stream));
     for (int a = 0; a < P->ndim; ++a)
         for (int s = 0; s < 2; ++s) {
-            HIP_TRY(hipMemsetAsync(P->psi[a][s][0], 0, (size_t)P->psi_elems[a] * P->nshots * P->esize, P->stream));
-            HIP_TRY(hipMemsetAsync(P->psi[a][s][1], 0, (size_t)P->psi_elems[a] * P->nshots * P->esize, P->stream));
-            if (P->xi_elems[a])
-                HIP_TRY(hipMemsetAsync(P->xi[a][s], 0, (size_t)P->xi_elems[a] * P->nshots * P->esize, P->stream));
+            const size_t pb = (size_t)P->psi_elems[a] * P->nshots * P->esize;
+            const size_t xb = (size_t)P->xi_elems[a] * P->nshots * P->esize;
+            HIP_TRY(hipMemsetAsync(P->psi[a][s][0], 0, pb, P->stream));
+            HIP_TRY(hipMemsetAsync(P->psi[a][s][1], 0, pb, P->stream));
+            if (P->psi_spare[a][s]) HIP_TRY(hipMemsetAsync(P->psi_spare[a][s], 0, pb, P->stream));
+            if (xb) HIP_TRY(hipMemsetAsync(P->xi[a][s], 0, xb, P->stream));
+            if (xb && P->xi_spare[a][s]) HIP_TRY(hipMemsetAsync(P->xi_spare[a][s], 0, xb, P->stream));
         }
+    P->boundary_dirty = false;
     if (P->d_traces && P->n_rec_total > 0 && P->nt > 0)
         HIP_TRY(hipMemsetAsync(P->d_traces, 0, (size_t)P->n_rec_total * P->nt * P->esize, P->stream));
     HIP_TRY(hipStreamSynchronize(P->stream));
@@ -920,20 +1170,14 @@ int awcpml_time_advance(awcpml_plan* P, int64_t nsteps, double* ms_total) {
     if (!ms_total || nsteps < 0) return fail(AWCPML_ERR_ARG, "bad argument");
     if ((st = ready(P))) return st;
     if (P->it + nsteps > P->nt) return fail(AWCPML_ERR_STATE, "advance past nt");
+    if (P->use_graph && nsteps > 0) {
+        // capture outside the timed region
+        GraphEntry* g = nullptr;
+        if ((st = plan_graph(P, nsteps, &g))) return st;
+    }
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    if (P->use_graph && nsteps > 0 && !find_graph(P->graphs, P->it, nsteps)) {
-        // capture outside the timed region
-        const int64_t it0 = P->it;
-        const int rot0 = P->rot, par0 = P->par;
-        hipGraphExec_t exec;
-        st = capture_steps(P->stream, nsteps, [&](int64_t q) { return do_step(P, it0 + q); }, &exec);
-        P->rot = rot0;
-        P->par = par0;
-        if (st) return st;
-        P->graphs.push_back({it0, nsteps, exec});
-    }
     HIP_TRY(hipEventRecord(e0, P->stream));
     st = run_steps(P, nsteps);
     HIP_TRY(hipEventRecord(e1, P->stream));
@@ -991,6 +1235,88 @@ int awcpml_get_shot_group(const awcpml_plan* P, int* shot_group) {
     return AWCPML_OK;
 }
 
+int awcpml_set_time_block(awcpml_plan* P, int steps, int block_rows, int march_len, int prefetch) {
+    g_err.clear();
+    int st = check_plan(P);
+    if (st) return st;
+    if (steps != 1 && steps != 2) return fail(AWCPML_ERR_ARG, "steps must be 1 or 2 (got %d)", steps);
+    if (steps == 2 && P->ndim != 3)
+        return fail(AWCPML_ERR_ARG, "two steps per launch are implemented for 3D plans");
+    if (block_rows != 0 && block_rows != 4 && block_rows != 8 && block_rows != 16)
+        return fail(AWCPML_ERR_ARG, "block_rows must be 0, 4, 8 or 16 (got %d)", block_rows);
+    if (march_len < 0) return fail(AWCPML_ERR_ARG, "march_len < 0");
+    if (prefetch < 0 || prefetch > 3) return fail(AWCPML_ERR_ARG, "prefetch must be 0..3 (got %d)", prefetch);
+    if (steps == 2 && !P->p_spare) {
+        // spares start as zeros, like every buffer after a reset; a spare is
+        // allocated before any two-step launch, so the layout is still home
+        const size_t fbytes = (size_t)P->field_elems * P->nshots * P->esize;
+        if ((st = dalloc(&P->p_spare, fbytes))) return st;
+        HIP_TRY(hipMemsetAsync(P->p_spare, 0, fbytes, P->stream));
+        for (int a = 0; a < P->ndim; ++a)
+            for (int s = 0; s < 2; ++s) {
+                const size_t pb = (size_t)P->psi_elems[a] * P->nshots * P->esize;
+                const size_t xb = (size_t)P->xi_elems[a] * P->nshots * P->esize;
+                if ((st = dalloc(&P->psi_spare[a][s], pb))) return st;
+                if ((st = dalloc(&P->xi_spare[a][s], xb))) return st;
+                HIP_TRY(hipMemsetAsync(P->psi_spare[a][s], 0, pb, P->stream));
+                if (xb) HIP_TRY(hipMemsetAsync(P->xi_spare[a][s], 0, xb, P->stream));
+            }
+        HIP_TRY(hipStreamSynchronize(P->stream));
+        P->home.p[3] = P->p_spare;
+        for (int a = 0; a < 3; ++a)
+            for (int s = 0; s < 2; ++s) {
+                P->home.psi[a][s][2] = P->psi_spare[a][s];
+                P->home.xi[a][s][1] = P->xi_spare[a][s];
+            }
+    }
+    P->time_block = steps;
+    P->tb_rows = block_rows;
+    P->tb_march = march_len;
+    P->tb_prefetch = prefetch;
+    drop_graphs(P->graphs);
+    return AWCPML_OK;
+}
+
+int awcpml_get_time_block(const awcpml_plan* P, int* steps, int* block_rows, int* march_len,
+                          int* prefetch) {
+    if (!P || !steps || !block_rows || !march_len || !prefetch)
+        return fail(AWCPML_ERR_ARG, "NULL argument");
+    int W, ml, pf;
+    two_tiling(P, &W, &ml, &pf);
+    // the steps a launch advances now (1 where two-step launches do not apply)
+    *steps = two_ok(P) ? 2 : 1;
+    *block_rows = W;
+    *march_len = ml;
+    *prefetch = pf;
+    return AWCPML_OK;
+}
+
+int awcpml_time_stencil2(awcpml_plan* P, int nlaunches, double* ms_total) {
+    g_err.clear();
+    int st = check_plan(P);
+    if (st) return st;
+    if (!ms_total || nlaunches < 1) return fail(AWCPML_ERR_ARG, "bad argument");
+    if ((st = ready(P))) return st;
+    if (!two_ok(P)) return fail(AWCPML_ERR_STATE, "two-step launches do not apply to this plan");
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, P->stream));
+    for (int q = 0; q < nlaunches; ++q) {
+        st = P->dtype == AWCPML_F64 ? launch_stencil2_t<double>(P, 0) : launch_stencil2_t<float>(P, 0);
+        if (st) break;
+        two_swap(P);
+    }
+    HIP_TRY(hipEventRecord(e1, P->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_total = ms;
+    return st;
+}
+
 int awcpml_batch_create(awcpml_plan* const* plans, int count, awcpml_batch** out) {
     g_err.clear();
     if (!out || !plans) return fail(AWCPML_ERR_ARG, "NULL argument");
@@ -1005,6 +1331,11 @@ int awcpml_batch_create(awcpml_plan* const* plans, int count, awcpml_batch** out
             return fail(AWCPML_ERR_ARG, "plans of a batch must share ndim and dtype (plan %d differs)", q);
         int st = ready(plans[q]);
         if (st) return st;
+        BufLayout now = get_layout(plans[q]);
+        now.rot = plans[q]->home.rot;
+        now.par = plans[q]->home.par;
+        if (!(now == plans[q]->home))
+            return fail(AWCPML_ERR_STATE, "plan %d ran two-step launches: reset it before batching", q);
         if (plans[q]->block_y != plans[0]->block_y || plans[q]->prefetch != plans[0]->prefetch)
             return fail(AWCPML_ERR_ARG, "plans of a batch must share block_y and prefetch (plan %d differs)", q);
     }

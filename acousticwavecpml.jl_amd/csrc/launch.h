@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "stencil2_kernels.h"
 #include "stencil_kernels.h"
 
 namespace awcpml {
@@ -23,5 +24,10 @@ void launch_plan_stencil(const StencilArgs<T>& A, const LaunchShape& L, int pipe
 
 template <typename T, int NDIM>
 void launch_batch_stencil(const BatchArgs<T>& A, const LaunchShape& L, int pipe, bool off64);
+
+// Two time steps per launch (3D plans); rows: waves per block (4, 8, 16);
+// pf: planes of loads in flight per wave (1..3).
+template <typename T>
+void launch_plan_stencil2(const Stencil2Args<T>& B, const LaunchShape& L, int rows, int pf);
 
 }  // namespace awcpml

@@ -1,0 +1,437 @@
+// stencil2_kernels.h -- two time steps per launch (temporal blocking) of the
+// 3D CPML pressure stencil.
+//
+// A launch takes (p^{t-1}, p^t) to (p^{t+1}, p^{t+2}).  Block (64, W): lane
+// -> x, wave -> y row, every wave marching z through one chunk of planes.
+// The block owns the nodes of its inner 62 lanes and W-2 rows; the outer
+// lanes and rows are a one-node halo on which only step t+1 is evaluated
+// (the neighbouring blocks evaluate it too), so that step t+2 finds all its
+// operands inside the block.  At march position k a wave
+//   1. evaluates step t+1 at plane k on all its lanes,
+//   2. puts that p^{t+1} plane row into a three-plane LDS ring (and stores
+//      it to HBM on the owned nodes: the next launch's pold, and the
+//      receivers sample it),
+//   3. after the block barrier evaluates step t+2 at plane k-1 on the
+//      owned nodes: x neighbours of p^{t+1} by lane shifts, y neighbours
+//      from the ring, z neighbours from the wave's own registers.
+// The chunk's march starts one plane early and ends one plane late.
+//
+// CPML: every slab node evaluates both psi edges of each of its axes in
+// step t+1 (as the single-step kernel's per-node form does) and keeps them
+// and its xi in registers for step t+2; only step t+2's psi and xi are
+// stored, into buffers distinct from the inputs, since other blocks read
+// the inputs on their halo nodes during the launch.  The arithmetic of each
+// step is the single-step kernel's, operation for operation, so two steps
+// of this kernel equal two single-step launches bit for bit.
+//
+// The host uses this kernel only when every field buffer is zero on the
+// Dirichlet nodes (no source on such a node since the last reset; the halo
+// lanes and rows read those nodes as 0), the fields take 32-bit byte
+// offsets and no d(pnew)/d(fact) output is requested.
+#pragma once
+
+#include <utility>
+
+#include "stencil_kernels.h"
+
+namespace awcpml {
+
+constexpr int kOwnX = kTileX - 2;  // owned x nodes per wave
+
+template <typename T>
+struct Stencil2Args {
+    // geometry, coefficients, pold = p^{t-1}, pcur = p^t; side[][].psi_rd
+    // and .xi are step t+1's inputs, side[][].psi_wr receives step t+2's psi
+    StencilArgs<T> s;
+    T* p1;                   // p^{t+1}
+    T* p2;                   // p^{t+2}
+    T* xi_out[3][2];         // xi after step t+2
+    const int64_t* src_lin;  // element offsets of the sources (all shots)
+    const T* src_wav;        // nt values per source
+    const int* src_start;    // nshots+1 CSR offsets
+    int64_t nt;
+    int64_t it0;             // 0-based index of step t+1
+};
+
+// Step t+1 results of one node that step t+2 at the same node reads.
+template <typename T>
+struct Mid {
+    T p1;           // p^{t+1}
+    T pc, f;        // p^t (step t+2's pold) and fact
+    T xl, xh, xx;   // x slab: psi^{t+1} of edges m-1 and m, xi^{t+1}[m-1]
+    T yl, yh, yx;   // y slab
+    T zl, zh, zx;   // z slab
+};
+
+// One axis's CPML terms at a slab node: the node's two psi edges from the
+// previous psi values (psi_old = psi[m], psi_old1 = psi[m-1]), xi[m-1]
+// from xi_old; returns dpsi + xi (acoustic_3D_xPU.jl update_psi_*! and the
+// damping term of update_p!).  Same operations as cpml_apply.
+template <typename T>
+__device__ __forceinline__ T cpml_pair(const LayerCoef<T>& c, T psi_old, T psi_old1, T xi_old,
+                                       T pm, T pc, T pp, T d2, T inv_d, T& lo, T& hi, T& xn) {
+    hi = c.bKh * psi_old + c.ah * (pp - pc) * inv_d;
+    lo = c.bKh1 * psi_old1 + c.ah1 * (pc - pm) * inv_d;
+    const T dpsi = (hi - lo) * inv_d;
+    xn = c.bK1 * xi_old + c.a1 * (d2 + dpsi);
+    return dpsi + xn;
+}
+
+// Step t+1 at one node from its loads.  ZS: z slab side of the plane (-1:
+// none), a template constant so that no per-side array is indexed at run
+// time (that would place the array in scratch memory).
+template <typename T, typename Off, int SA, bool SB, int ZS>
+__device__ __forceinline__ Mid<T> two_first(const March<T, Off, 3>& M, int mz,
+                                            const NodeLoads<T>& q) {
+    Mid<T> r = {};
+    const T two = T(2);
+    const T pc = q.pc, f = q.f;
+    const T d2a = (q.xp - two * pc + q.xm) * M.i2a;
+    const T d2b = (q.yp - two * pc + q.ym) * M.i2b;
+    const T d2m = (q.pp - two * pc + q.pm) * M.i2m;
+    T damp = T(0);
+    if (SA == 2 && M.sa.in)
+        damp += f * cpml_pair(M.sa.c, q.psa0, q.psa1, q.xia, q.xm, pc, q.xp, d2a, M.ia, r.xl, r.xh,
+                              r.xx);
+    if (SB)
+        damp += f * cpml_pair(M.sb.c, q.psb0, q.psb1, q.xib, q.ym, pc, q.yp, d2b, M.ib, r.yl, r.yh,
+                              r.yx);
+    if (ZS >= 0) {
+        const LayerCoef<T> cz = lds_coef(M.lds, M.h1, ZS, mz);
+        damp += f * cpml_pair(cz, q.psm0, q.psm1, q.xim, q.pm, pc, q.pp, d2m, M.imm, r.zl, r.zh,
+                              r.zx);
+    }
+    const T lap = d2a + d2b + d2m;  // (d2x + d2y) + d2z
+    r.p1 = two * pc - q.po + f * lap + damp;
+    r.pc = pc;
+    r.f = f;
+    return r;
+}
+
+// Output pointers of one launch for one shot.
+template <typename T, typename Off>
+struct TwoOut {
+    T* p1;
+    T* p2;
+    T* xi_x;      // this column's x slab side (valid when M.sa.in)
+    T* xi_y;      // this row's y slab side (valid when M.sb.in)
+    T* xi_z[2];   // z slab sides
+};
+
+// Step t+2 at one node (plane k) from its step t+1 results d and the
+// neighbouring p^{t+1} values; stores on owned nodes only.
+template <typename T, typename Off, int SA, bool SB, int ZS>
+__device__ __forceinline__ void two_second(const March<T, Off, 3>& M, const TwoOut<T, Off>& O,
+                                           const Mid<T>& d, T pm, T pp, T xm, T xp, T ym, T yp,
+                                           bool own, int mz, int k) {
+    const T two = T(2);
+    const T pc = d.p1, f = d.f;
+    const T d2a = (xp - two * pc + xm) * M.i2a;
+    const T d2b = (yp - two * pc + ym) * M.i2b;
+    const T d2m = (pp - two * pc + pm) * M.i2m;
+    T damp = T(0);
+    if (SA == 2 && M.sa.in) {
+        T lo, hi, xn;
+        damp += f * cpml_pair(M.sa.c, d.xh, d.xl, d.xx, xm, pc, xp, d2a, M.ia, lo, hi, xn);
+        if (own) {
+            stg(M.sa.p.psi_wr, M.sa.o_psi + (Off)k * M.sa.st_psi, hi);
+            if (M.sa.m == 1) stg(M.sa.p.psi_wr, M.sa.o_psi1 + (Off)k * M.sa.st_psi, lo);
+            stg(O.xi_x, M.sa.o_xi + (Off)k * M.sa.st_xi, xn);
+        }
+    }
+    if (SB) {
+        T lo, hi, xn;
+        damp += f * cpml_pair(M.sb.c, d.yh, d.yl, d.yx, ym, pc, yp, d2b, M.ib, lo, hi, xn);
+        if (own) {
+            stg(M.sb.p.psi_wr, M.sb.o_psi + (Off)k * M.sb.st_psi, hi);
+            if (M.sb.m == 1) stg(M.sb.p.psi_wr, M.sb.o_psi1 + (Off)k * M.sb.st_psi, lo);
+            stg(O.xi_y, M.sb.o_xi + (Off)k * M.sb.st_xi, xn);
+        }
+    }
+    if (ZS >= 0) {
+        const LayerCoef<T> cz = lds_coef(M.lds, M.h1, ZS, mz);
+        T lo, hi, xn;
+        damp += f * cpml_pair(cz, d.zh, d.zl, d.zx, pm, pc, pp, d2m, M.imm, lo, hi, xn);
+        if (own) {
+            T* pw = M.pm_side[ZS < 0 ? 0 : ZS].psi_wr;
+            stg(pw, M.col + (Off)mz * M.sm, hi);
+            if (mz == 1) stg(pw, M.col + (Off)(mz - 1) * M.sm, lo);
+            stg(O.xi_z[ZS < 0 ? 0 : ZS], M.col + (Off)(mz - 1) * M.sm, xn);
+        }
+    }
+    const T lap = d2a + d2b + d2m;
+    if (own) stg_stream(O.p2, M.col + (Off)k * M.sm, two * pc - d.pc + f * lap + damp);
+}
+
+// Step t+1 loads of plane k, dispatched on the plane's z slab side.
+template <typename T, typename Off, int SA, bool SB, bool ZM>
+__device__ __forceinline__ void two_load(const StencilArgs<T>& A, const March<T, Off, 3>& M,
+                                         int k, NodeLoads<T>& q) {
+    if (!ZM) {
+        load_node<T, Off, 3, SA, SB, false>(M, k, 0, q);
+        return;
+    }
+    int mz;
+    const int zs = classify(k, A.nz, A.halo, A.active[2], &mz);
+    if (zs == 0) load_node<T, Off, 3, SA, SB, true>(M, k, 0, q);
+    else if (zs == 1) load_node<T, Off, 3, SA, SB, true>(M, k, 1, q);
+    else load_node<T, Off, 3, SA, SB, false>(M, k, 0, q);
+}
+
+template <typename T, typename Off, int SA, bool SB, bool ZM>
+__device__ __forceinline__ Mid<T> two_first_at(const StencilArgs<T>& A, const March<T, Off, 3>& M,
+                                               int k, const NodeLoads<T>& q) {
+    if (!ZM) return two_first<T, Off, SA, SB, -1>(M, 0, q);
+    int mz;
+    const int zs = classify(k, A.nz, A.halo, A.active[2], &mz);
+    if (zs == 0) return two_first<T, Off, SA, SB, 0>(M, mz, q);
+    if (zs == 1) return two_first<T, Off, SA, SB, 1>(M, mz, q);
+    return two_first<T, Off, SA, SB, -1>(M, mz, q);
+}
+
+template <typename T, typename Off, int SA, bool SB, bool ZM>
+__device__ __forceinline__ void two_second_at(const StencilArgs<T>& A, const March<T, Off, 3>& M,
+                                              const TwoOut<T, Off>& O, const Mid<T>& d, T pm,
+                                              T pp, T xm, T xp, T ym, T yp, bool own, int k) {
+    if (!ZM) {
+        two_second<T, Off, SA, SB, -1>(M, O, d, pm, pp, xm, xp, ym, yp, own, 0, k);
+        return;
+    }
+    int mz;
+    const int zs = classify(k, A.nz, A.halo, A.active[2], &mz);
+    if (zs == 0) two_second<T, Off, SA, SB, 0>(M, O, d, pm, pp, xm, xp, ym, yp, own, mz, k);
+    else if (zs == 1) two_second<T, Off, SA, SB, 1>(M, O, d, pm, pp, xm, xp, ym, yp, own, mz, k);
+    else two_second<T, Off, SA, SB, -1>(M, O, d, pm, pp, xm, xp, ym, yp, own, mz, k);
+}
+
+// Per-wave march state: the ring of step t+1 loads (PF planes in flight),
+// step t+1's results of the previous plane, p^{t+1} two planes back, and
+// the LDS ring slot of the current plane.
+template <typename T, int PF>
+struct TwoCtx {
+    NodeLoads<T> qr[PF];
+    Mid<T> d;
+    T p1m;
+    int slot;
+};
+
+// Uniform per-launch values of a wave's march.
+struct TwoGeom {
+    int wy, kbeg, kend, nz, ksrc_lo, ksrc_hi, s0, s1;
+    int64_t col_el, plane;
+    bool in1, own;
+};
+
+// One march position k: step t+1 at plane k, refill of ring slot U with
+// plane k+PF, the LDS exchange, step t+2 at plane k-1.  GEN = false: k is
+// in the steady state, where every plane index condition holds (computed,
+// refilled, owned plane, step t+2 due), so none is tested.
+template <typename T, typename Off, int W, int PF, int SA, bool SB, bool ZM, bool GEN, int U>
+__device__ __forceinline__ void two_iter(const Stencil2Args<T>& B, const March<T, Off, 3>& M,
+                                         const TwoOut<T, Off>& O, T (*ring)[W][kTileX],
+                                         const TwoGeom& G, TwoCtx<T, PF>& C, int k) {
+    const StencilArgs<T>& A = B.s;
+    const int lane = (int)threadIdx.x;
+    Mid<T> dn = {};
+    if (!GEN || (k >= 1 && k <= G.nz - 2)) {
+        dn = two_first_at<T, Off, SA, SB, ZM>(A, M, k, C.qr[U]);
+        // inject_sources! of step t+1, in source order
+        if (k >= G.ksrc_lo && k <= G.ksrc_hi) {
+            const int64_t lin = G.col_el + (int64_t)k * G.plane;
+            for (int e = G.s0; e < G.s1; ++e)
+                if (B.src_lin[e] == lin) dn.p1 += B.src_wav[(int64_t)e * B.nt + B.it0];
+        }
+        dn.p1 = G.in1 ? dn.p1 : T(0);
+    }
+    const int kr = k + PF;
+    if (!GEN || (kr <= G.kend && kr <= G.nz - 2)) two_load<T, Off, SA, SB, ZM>(A, M, kr, C.qr[U]);
+    __builtin_amdgcn_sched_barrier(0);
+    ring[C.slot][G.wy][lane] = dn.p1;
+    if (G.own && (!GEN || (k >= G.kbeg && k < G.kend)))
+        stg_stream(O.p1, M.col + (Off)k * M.sm, dn.p1);
+    __syncthreads();
+    if (!GEN || k - 1 >= G.kbeg) {
+        const int sp = C.slot == 0 ? 2 : C.slot - 1;  // plane k-1
+        const T xm = __shfl_up(C.d.p1, 1u, kTileX);
+        const T xp = __shfl_down(C.d.p1, 1u, kTileX);
+        T ym = T(0), yp = T(0);
+        if (G.wy >= 1 && G.wy <= W - 2) {
+            ym = ring[sp][G.wy - 1][lane];
+            yp = ring[sp][G.wy + 1][lane];
+        }
+        two_second_at<T, Off, SA, SB, ZM>(A, M, O, C.d, C.p1m, dn.p1, xm, xp, ym, yp, G.own, k - 1);
+    }
+    C.p1m = C.d.p1;
+    C.d = dn;
+    C.slot = C.slot == 2 ? 0 : C.slot + 1;
+}
+
+// Positions k+u, u in Us..., in order, ring slot u % PF (a constant), each
+// only while k+u <= kend when CHECK.
+template <typename T, typename Off, int W, int PF, int SA, bool SB, bool ZM, bool GEN, bool CHECK,
+          int... Us>
+__device__ __forceinline__ void two_group(const Stencil2Args<T>& B, const March<T, Off, 3>& M,
+                                          const TwoOut<T, Off>& O, T (*ring)[W][kTileX],
+                                          const TwoGeom& G, TwoCtx<T, PF>& C, int k,
+                                          std::integer_sequence<int, Us...>) {
+    ((!CHECK || k + Us <= G.kend
+          ? (two_iter<T, Off, W, PF, SA, SB, ZM, GEN, Us % PF>(B, M, O, ring, G, C, k + Us), 0)
+          : 0),
+     ...);
+}
+
+// The march of one wave over planes kbeg-1 .. kend (step t+1) and
+// kbeg .. kend-1 (step t+2): a head of H generic positions (H: 2 rounded
+// up to a multiple of PF, so that the steady state starts at ring slot 0),
+// the steady state unrolled PF times (constant slot indices), and a
+// generic tail.  Every wave of the block runs the same positions (kbeg,
+// kend are block-uniform) and so meets every barrier.
+template <typename T, typename Off, int W, int PF, int SA, bool SB, bool ZM>
+__device__ __forceinline__ void two_march(const Stencil2Args<T>& B, const March<T, Off, 3>& M,
+                                          const TwoOut<T, Off>& O, T (*ring)[W][kTileX],
+                                          const TwoGeom& G) {
+    const StencilArgs<T>& A = B.s;
+    TwoCtx<T, PF> C;
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+        C.qr[u] = NodeLoads<T>{};
+        const int kk = G.kbeg - 1 + u;
+        if (kk >= 1 && kk <= G.kend && kk <= G.nz - 2) two_load<T, Off, SA, SB, ZM>(A, M, kk, C.qr[u]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    C.d = Mid<T>{};
+    C.p1m = T(0);
+    C.slot = (G.kbeg - 1) % 3;
+    constexpr int H = (2 + PF - 1) / PF * PF;
+    int k = G.kbeg - 1;
+    two_group<T, Off, W, PF, SA, SB, ZM, true, true>(B, M, O, ring, G, C, k,
+                                                     std::make_integer_sequence<int, H>{});
+    k += H;
+    const int last = (G.kend < G.nz - 2 ? G.kend : G.nz - 2) - PF;  // last steady position
+    for (; k + PF - 1 <= last; k += PF)
+        two_group<T, Off, W, PF, SA, SB, ZM, false, false>(B, M, O, ring, G, C, k,
+                                                           std::make_integer_sequence<int, PF>{});
+    for (; k <= G.kend; k += PF)
+        two_group<T, Off, W, PF, SA, SB, ZM, true, true>(B, M, O, ring, G, C, k,
+                                                         std::make_integer_sequence<int, PF>{});
+}
+
+template <typename T, typename Off, int W, int PF>
+__global__ __launch_bounds__(64 * W) void stencil3d_two(const Stencil2Args<T> B) {
+    __shared__ T ring[3][W][kTileX];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* lds = reinterpret_cast<T*>(smem_raw);
+    const StencilArgs<T>& A = B.s;
+    const int lb = logical_block(blockIdx.x, A.grid_blocks);
+    if (lb >= A.logical_blocks) return;  // whole block
+    if (A.active[2][0] || A.active[2][1]) stage_march_coef(A, 2, lds);
+    __syncthreads();
+
+    // logical order: x tile fastest, then y tile, chunk, shot
+    int rest = lb;
+    const int tx = rest % A.tiles_x; rest /= A.tiles_x;
+    const int ty = rest % A.tiles_y; rest /= A.tiles_y;
+    const int chunk = rest % A.nchunks;
+    const int shot = rest / A.nchunks;
+    const int wy = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    const int lane = (int)threadIdx.x;
+    const int64_t nx = A.nx, ny = A.ny, nz = A.nz, halo = A.halo;
+    const int64_t i = (int64_t)tx * kOwnX + lane;
+    const int64_t j = (int64_t)ty * (W - 2) + wy;
+    // clamped to the interior so every load stays inside the field
+    const int64_t ic = i < 1 ? 1 : (i > nx - 2 ? nx - 2 : i);
+    const int64_t jc = j < 1 ? 1 : (j > ny - 2 ? ny - 2 : j);
+    const bool in1 = i == ic && j == jc;
+    const bool own = in1 && lane >= 1 && lane <= kOwnX && wy >= 1 && wy <= W - 2;
+    const int kbeg = 1 + chunk * A.march_len;
+    const int kend = (int)min((int64_t)kbeg + A.march_len, nz - 1);
+
+    constexpr Off E = (Off)sizeof(T);
+    const int64_t fo = (int64_t)shot * A.field_stride;
+    March<T, Off, 3> M;
+    M.pcur = A.pcur + fo;
+    M.pold = A.pold + fo;
+    M.pnew = nullptr;
+    M.fact = A.fact;
+    M.qout = nullptr;
+    M.sy = (Off)A.px * E;
+    M.sm = (Off)(A.px * ny) * E;
+    M.col = ((Off)ic + (Off)A.px * (Off)jc) * E;
+    M.i2a = A.inv_d2[0];
+    M.i2b = A.inv_d2[1];
+    M.i2m = A.inv_d2[2];
+    M.ia = A.inv_d[0];
+    M.ib = A.inv_d[1];
+    M.imm = A.inv_d[2];
+    M.lds = lds;
+    M.h1 = (int)halo + 1;
+    M.store_p = true;
+    const Off hp1 = (Off)halo + 1, hl = (Off)halo;
+    // psi_x: (halo+1) x ny x nz, xi_x: halo x ny x nz
+    col_slab<T, Off>(A, 0, ic, nx, shot, hp1 * (Off)jc * E, E, hl * (Off)jc * E, E,
+                     hp1 * (Off)ny * E, hl * (Off)ny * E, M.sa);
+    // psi_y: px x (halo+1) x nz, xi_y: px x halo x nz
+    col_slab<T, Off>(A, 1, jc, ny, shot, (Off)ic * E, M.sy, (Off)ic * E, M.sy, hp1 * M.sy,
+                     hl * M.sy, M.sb);
+    for (int s = 0; s < 2; ++s) M.pm_side[s] = slab_ptrs(A, 2, s, shot);
+    M.mshift[0] = 0;
+    M.mshift[1] = (int)(nz - halo - 2);
+
+    TwoOut<T, Off> O;
+    O.p1 = B.p1 + fo;
+    O.p2 = B.p2 + fo;
+    int mx, my;
+    const int sx = classify(ic, nx, halo, A.active[0], &mx);
+    const int sy = classify(jc, ny, halo, A.active[1], &my);
+    O.xi_x = (sx == 1 ? B.xi_out[0][1] : B.xi_out[0][0]) + shot * A.xi_stride[0];
+    O.xi_y = (sy == 1 ? B.xi_out[1][1] : B.xi_out[1][0]) + shot * A.xi_stride[1];
+    O.xi_z[0] = B.xi_out[2][0] + shot * A.xi_stride[2];
+    O.xi_z[1] = B.xi_out[2][1] + shot * A.xi_stride[2];
+
+    // planes holding this shot's sources
+    TwoGeom G;
+    G.ksrc_lo = 1 << 30;
+    G.ksrc_hi = -1;
+    G.plane = A.px * ny;
+    G.s0 = B.src_start[shot];
+    G.s1 = B.src_start[shot + 1];
+    for (int e = G.s0; e < G.s1; ++e) {
+        const int kk = (int)(B.src_lin[e] / G.plane);
+        G.ksrc_lo = kk < G.ksrc_lo ? kk : G.ksrc_lo;
+        G.ksrc_hi = kk > G.ksrc_hi ? kk : G.ksrc_hi;
+    }
+    G.col_el = ic + A.px * jc;
+    G.wy = wy;
+    G.kbeg = kbeg;
+    G.kend = kend;
+    G.nz = (int)nz;
+    G.in1 = in1;
+    G.own = own;
+
+    const bool xs = __any(M.sa.in);
+    const bool ys = M.sb.in;  // wave-uniform
+    // any z slab layer among the planes this chunk reads (block-uniform)
+    const bool zm = (A.active[2][0] && kbeg - 1 <= halo) || (A.active[2][1] && kend >= nz - halo - 1);
+#define AWCPML_TWO(SA_, SB_, ZM_)                                                             \
+    two_march<T, Off, W, PF, SA_, SB_, ZM_>(B, M, O, ring, G)
+    if (zm) {
+        if (xs) {
+            if (ys) AWCPML_TWO(2, true, true);
+            else AWCPML_TWO(2, false, true);
+        } else {
+            if (ys) AWCPML_TWO(0, true, true);
+            else AWCPML_TWO(0, false, true);
+        }
+    } else {
+        if (xs) {
+            if (ys) AWCPML_TWO(2, true, false);
+            else AWCPML_TWO(2, false, false);
+        } else {
+            if (ys) AWCPML_TWO(0, true, false);
+            else AWCPML_TWO(0, false, false);
+        }
+    }
+#undef AWCPML_TWO
+}
+
+}  // namespace awcpml

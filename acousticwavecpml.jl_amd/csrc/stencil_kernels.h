@@ -877,12 +877,13 @@ struct IoArgs {
     T* traces;               // nt values per receiver
     const int* rec_start;
     int64_t nt;
+    int inject;              // 0: receivers only (the sources are already in pnew)
 };
 
 template <typename T>
 __device__ __forceinline__ void io_block(const IoArgs<T>& A, int shot, int64_t it0) {
     T* p = A.pnew + (int64_t)shot * A.field_stride;
-    const int s0 = A.src_start[shot], s1 = A.src_start[shot + 1];
+    const int s0 = A.src_start[shot], s1 = A.inject ? A.src_start[shot + 1] : s0;
     if (A.src_seq[shot]) {
         if (threadIdx.x == 0)
             for (int e = s0; e < s1; ++e) p[A.src_lin[e]] += A.src_wav[(int64_t)e * A.nt + it0];

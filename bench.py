@@ -83,20 +83,31 @@ def time_variant(S, dtype, nshots, steps, warmup, freetop, kernel_launches=50, t
         P.advance(S.nt)
     t = (time.perf_counter() - t0) / max(steps, 1)
     traces0 = P.traces(0)
-    # dominant kernel alone, HIP events on the plan's stream
+    # dominant kernel alone, HIP events on the plan's stream: the two-step
+    # kernel (two time steps per launch) when the tuner chose it
+    tb = P.time_block()
     P.reset()
-    ms = P.time_stencil(kernel_launches) / kernel_launches
+    if tb[0] == 2:
+        nl = max(1, kernel_launches // 2)
+        ms = P.time_stencil2(nl) / nl
+    else:
+        ms = P.time_stencil(kernel_launches) / kernel_launches
     tiling = P.tiling() + (P.shot_group(),)
     P.close()
     B, N = algorithmic_bytes(S.n, S.halo, freetop, esize, nshots)
+    B_launch = B * tb[0]
     pts = nshots * N * S.nt
     return {
         "dtype": dtype, "shots": nshots, "s_per_solve": t,
         "shot_gpts_per_s": pts / t / 1e9,
         "effective_gbs": B * S.nt / t / 1e9,
-        "kernel_ms": ms, "kernel_gbs": B / (ms * 1e-3) / 1e9,
-        "bytes_per_launch": B, "tiling": {"block_y": tiling[0], "march_len": tiling[1], "prefetch": tiling[2],
+        "kernel": "stencil3d_two" if tb[0] == 2 else "stencil3d",
+        "steps_per_launch": tb[0],
+        "kernel_ms": ms, "kernel_gbs": B_launch / (ms * 1e-3) / 1e9,
+        "bytes_per_launch": B_launch,
+        "tiling": {"block_y": tiling[0], "march_len": tiling[1], "prefetch": tiling[2],
                    "shot_group": tiling[3]},
+        "time_block": {"steps": tb[0], "block_rows": tb[1], "march_len": tb[2], "prefetch": tb[3]},
         "_traces0": traces0,
     }
 
@@ -292,7 +303,7 @@ def load_traffic(kernel, workload):
     return None
 
 
-def profile_kernel(S, dtype, nshots, tiling, launches):
+def profile_kernel(S, dtype, nshots, tiling, launches, time_block=None):
     """Profiler target: only the stencil launches of one variant at a fixed
     launch shape (no autotuning, so every dispatch of the kernel is the same
     shape), then a 1 GiB FP32 device copy (1 GiB read + 1 GiB written) that
@@ -303,11 +314,17 @@ def profile_kernel(S, dtype, nshots, tiling, launches):
         P.set_tiling(*tiling[:3])
         if len(tiling) > 3:
             P.set_shot_group(tiling[3])
+    steps = 1
+    if time_block:
+        P.set_time_block(2, *time_block[:3])
+        steps = 2
     P.reset()
-    ms = P.time_stencil(launches) / launches
+    ms = (P.time_stencil2(launches) if steps == 2 else P.time_stencil(launches)) / launches
     esize = 8 if dtype == "f64" else 4
     B, _ = algorithmic_bytes(S.n, S.halo, True, esize, nshots)
-    print(f"[profile] {dtype} S={nshots} tiling={P.tiling()} shot_group={P.shot_group()} {launches} launches "
+    B *= steps
+    print(f"[profile] {dtype} S={nshots} tiling={P.tiling()} shot_group={P.shot_group()} "
+          f"time_block={P.time_block()} {launches} launches "
           f"{ms:.4f} ms/launch, {B} algorithmic bytes/launch, {B / ms / 1e6:.1f} GB/s", flush=True)
     P.close()
     n = 1 << 28
@@ -335,6 +352,8 @@ def main():
     ap.add_argument("--profile-dtype", default="f32", choices=["f32", "f64"])
     ap.add_argument("--tiling", default="",
                     help="block_y,march_len,prefetch[,shot_group] for --profile-kernel")
+    ap.add_argument("--time-block", default="",
+                    help="block_rows,march_len,prefetch: --profile-kernel times two-step launches")
     ap.add_argument("--launches", type=int, default=20)
     ap.add_argument("--profile-fwi", action="store_true",
                     help="profiler target: one FP32 FWI gradient of the 554^2 32-shot workload")
@@ -374,7 +393,8 @@ def main():
     if args.profile_kernel:
         tiling = tuple(int(v) for v in args.tiling.split(",")) if args.tiling else None
         nsh = args.shots if args.profile_dtype == "f32" else 1
-        profile_kernel(S, args.profile_dtype, nsh, tiling, args.launches)
+        tb = tuple(int(v) for v in args.time_block.split(",")) if args.time_block else None
+        profile_kernel(S, args.profile_dtype, nsh, tiling, args.launches, tb)
         return
 
     v64 = time_variant(S, "f64", 1, args.steps, args.warmup, case["freetop"])
@@ -387,7 +407,7 @@ def main():
     fp32_err = rel_l2(v32["_traces0"], v64["_traces0"])
 
     main_v = vb
-    kname = "stencil3d<float>"
+    kname = ("stencil3d_two<float>" if main_v["steps_per_launch"] == 2 else "stencil3d<float>")
     workload = f"plain3d_{S.n[0]}^3_nt{S.nt}_f32_S{args.shots}"
     achieved = main_v["kernel_gbs"]
     traffic = load_traffic(kname, workload)
@@ -415,6 +435,7 @@ def main():
             "fp32_single": {k: (round(v, 4) if isinstance(v, float) else v)
                             for k, v in v32.items() if not k.startswith("_")},
             "tiling": main_v["tiling"],
+            "time_block": main_v["time_block"],
             "fp32_vs_fp64_plain": round(main_v["shot_gpts_per_s"] / v64["shot_gpts_per_s"], 3),
             "fp32_trace_rel_l2_vs_fp64": fp32_err,
         },
@@ -422,6 +443,7 @@ def main():
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": kname, "bytes_per_launch": main_v["bytes_per_launch"],
+            "steps_per_launch": main_v["steps_per_launch"],
             "kernel_ms": round(main_v["kernel_ms"], 5),
         },
         "cpu_baseline": None,

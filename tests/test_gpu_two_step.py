@@ -1,0 +1,174 @@
+"""GPU tests of two time steps per stencil launch (awcpml_set_time_block).
+
+A two-step launch evaluates each step with the single-step kernel's
+arithmetic, so every result must equal one step per launch bit for bit,
+in FP64 and FP32: traces, and the three field buffers afterwards.  The
+FP64 results are also compared with the CPU oracle bit for bit.  Cases
+cover several x tiles (62 owned nodes per wave) and y tiles (rows-2 owned
+rows per block), chunk lengths that leave short chunks, odd step counts
+(the last step runs alone), several shots, sources on different planes and
+two sources on one node, graphs, and the fallback to single steps for a
+source on a Dirichlet node.
+"""
+import numpy as np
+import pytest
+
+import awcpml
+from cases import oracle_problem, small3d
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle(case):
+    o = oracle_problem(case)
+    o._spacing = [case["l"][a] / (case["n"][a] - 1) for a in range(len(case["n"]))]
+    return o
+
+
+def _plan(o, dtype="f64", nshots=1, srcs=None, wav=None):
+    P = awcpml.Plan(o.n, o.halo, dtype=dtype, nshots=nshots)
+    sp = o._spacing
+    P.set_model(o.fact(), [1 / d for d in sp], [1 / (d * d) for d in sp])
+    for a in range(o.ndim):
+        P.set_cpml(a, [o.coef(a, w) for w in range(8)])
+    ps, pr = o.positions()
+    for s in range(nshots):
+        P.set_sources(s, ps if srcs is None else srcs[s], o.dt2srctf() if wav is None else wav)
+        P.set_receivers(s, pr)
+    return P
+
+
+def _state(P, nshots):
+    return [(P.traces(s), P.field(s, 0), P.field(s, 1), P.field(s, 2)) for s in range(nshots)]
+
+
+def _run(P, nt, steps, rows=0, march=0, splits=None):
+    P.set_time_block(steps, rows, march)
+    P.reset()
+    if splits:
+        done = 0
+        for n in splits:
+            P.advance(n)
+            done += n
+        P.advance(nt - done)
+    else:
+        P.advance(nt)
+
+
+def _same(a, b):
+    for x, y in zip(a, b):
+        for u, v in zip(x, y):
+            assert np.array_equal(u, v)
+
+
+CASES = {
+    "freetop": dict(n=(37, 29, 23), halo=5, freetop=True, nt=61),
+    "open_2src": dict(n=(37, 29, 23), halo=5, freetop=False, nt=57, nsrc=2),
+    "xtiles": dict(n=(150, 33, 26), halo=6, freetop=True, nt=45),
+    "ytiles": dict(n=(40, 71, 24), halo=4, freetop=False, nt=39),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("rows,march", [(8, 0), (4, 5), (16, 7)])
+def test_two_step_fp64_bitwise(name, rows, march):
+    print("[gpu] two-step fp64 %s rows %d march %d" % (name, rows, march), flush=True)
+    c = CASES[name]
+    case = small3d(n=c["n"], halo=c["halo"], freetop=c["freetop"], nt=c["nt"],
+                   nsrc=c.get("nsrc", 1))
+    o = _oracle(case)
+    o.run()
+    P = _plan(o)
+    _run(P, o.nt, 1)
+    one = _state(P, 1)
+    _run(P, o.nt, 2, rows, march)
+    assert P.time_block()[0] == 2
+    two = _state(P, 1)
+    _same(two, one)
+    assert np.array_equal(two[0][0], o.traces())
+    assert np.array_equal(two[0][2], o.pend())
+    P.close()
+
+
+@pytest.mark.parametrize("rows", [8, 16])
+def test_two_step_fp32_bitwise(rows):
+    print("[gpu] two-step fp32 rows %d" % rows, flush=True)
+    case = small3d(n=(150, 40, 29), halo=6, freetop=True, nt=70, nsrc=2)
+    o = _oracle(case)
+    P = _plan(o, dtype="f32")
+    _run(P, o.nt, 1)
+    one = _state(P, 1)
+    _run(P, o.nt, 2, rows, 0)
+    two = _state(P, 1)
+    _same(two, one)
+    P.close()
+
+
+def test_two_step_shots_graphs_splits():
+    print("[gpu] two-step shots + graphs + split advances", flush=True)
+    case = small3d(n=(70, 35, 27), halo=5, freetop=False, nt=64)
+    o = _oracle(case)
+    ps, _ = o.positions()
+    srcs = []
+    for d in (0, 4, -5):
+        q = ps.copy()
+        q[:, 0] += d
+        srcs.append(q)
+    P = _plan(o, nshots=3, srcs=srcs)
+    _run(P, o.nt, 1)
+    one = _state(P, 3)
+    _run(P, o.nt, 2, 8, 6, splits=[5, 20])  # odd first chunk: single steps interleave
+    _same(_state(P, 3), one)
+    P.set_graph(True)
+    for rep in range(3):
+        _run(P, o.nt, 2, 8, 6)
+        _same(_state(P, 3), one)
+        _run(P, o.nt, 2, 8, 6, splits=[7])
+        _same(_state(P, 3), one)
+    P.set_graph(False)
+    P.close()
+
+
+def test_two_step_sources_on_one_node():
+    print("[gpu] two-step two sources on one node", flush=True)
+    case = small3d(n=(37, 29, 23), halo=5, freetop=True, nt=50)
+    o = _oracle(case)
+    ps, _ = o.positions()
+    w = o.dt2srctf()
+    wav = np.asfortranarray(np.concatenate([w, 0.37 * w[::-1]], axis=1))
+    P = _plan(o, srcs=[np.concatenate([ps, ps], axis=0)], wav=wav)
+    _run(P, o.nt, 1)
+    one = _state(P, 1)
+    _run(P, o.nt, 2, 8, 0)
+    assert P.time_block()[0] == 2
+    _same(_state(P, 1), one)
+    P.close()
+
+
+def test_two_step_falls_back_for_dirichlet_source():
+    print("[gpu] two-step fallback: source on a Dirichlet node", flush=True)
+    case = small3d(n=(37, 29, 23), halo=5, freetop=True, nt=40)
+    o = _oracle(case)
+    ps, _ = o.positions()
+    q = ps.copy()
+    q[:, 2] = 1  # 1-based z index 1: the top boundary plane
+    P = _plan(o, srcs=[q])
+    _run(P, o.nt, 1)
+    one = _state(P, 1)
+    _run(P, o.nt, 2, 8, 0)
+    assert P.time_block()[0] == 1
+    _same(_state(P, 1), one)
+    P.close()
+
+
+def test_time_stencil2():
+    print("[gpu] time_stencil2", flush=True)
+    case = small3d(n=(70, 35, 27), halo=5, freetop=True, nt=20)
+    o = _oracle(case)
+    P = _plan(o, dtype="f32")
+    P.set_time_block(2)
+    P.reset()
+    assert P.time_stencil2(3) > 0.0
+    P.reset()
+    P.advance(o.nt)
+    P.close()
