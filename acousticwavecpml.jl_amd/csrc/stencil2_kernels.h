@@ -12,7 +12,7 @@
 //      it to HBM on the owned nodes: the next launch's pold, and the
 //      receivers sample it),
 //   3. after the block barrier evaluates step t+2 at plane k-1 on the
-//      owned nodes: x neighbours of p^{t+1} by lane shifts, y neighbours
+//      owned nodes: x neighbours of p^{t+1} by DPP lane shifts, y neighbours
 //      from the ring, z neighbours from the wave's own registers.
 // The chunk's march starts one plane early and ends one plane late.
 //
@@ -53,6 +53,23 @@ struct Stencil2Args {
     int64_t it0;             // 0-based index of step t+1
 };
 
+// Lane shifts across the 64-lane wavefront with DPP (VALU moves, no LDS
+// traffic): from_below gives lane i the value of lane i-1, from_above the
+// value of lane i+1; lanes without a neighbour keep their own value.
+__device__ __forceinline__ int dpp_shift(int v, bool below) {
+    return below ? __builtin_amdgcn_update_dpp(v, v, 0x138 /* wave_shr:1 */, 0xf, 0xf, false)
+                 : __builtin_amdgcn_update_dpp(v, v, 0x130 /* wave_shl:1 */, 0xf, 0xf, false);
+}
+__device__ __forceinline__ float lane_shift(float v, bool below) {
+    return __builtin_bit_cast(float, dpp_shift(__builtin_bit_cast(int, v), below));
+}
+__device__ __forceinline__ double lane_shift(double v, bool below) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = dpp_shift((int)(unsigned)(u & 0xffffffffull), below);
+    const int hi = dpp_shift((int)(unsigned)(u >> 32), below);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
 // Step t+1 results of one node that step t+2 at the same node reads.
 template <typename T>
 struct Mid {
@@ -65,16 +82,25 @@ struct Mid {
 
 // One axis's CPML terms at a slab node: the node's two psi edges from the
 // previous psi values (psi_old = psi[m], psi_old1 = psi[m-1]), xi[m-1]
-// from xi_old; returns dpsi + xi (acoustic_3D_xPU.jl update_psi_*! and the
-// damping term of update_p!).  Same operations as cpml_apply.
+// from xi_old, and t = dpsi + xi (acoustic_3D_xPU.jl update_psi_*! and the
+// damping term of update_p!).  Same operations as cpml_apply.  Returned by
+// value (out-parameters of the side-dispatched callers would be merged
+// into stores through selected addresses and kept in scratch memory).
 template <typename T>
-__device__ __forceinline__ T cpml_pair(const LayerCoef<T>& c, T psi_old, T psi_old1, T xi_old,
-                                       T pm, T pc, T pp, T d2, T inv_d, T& lo, T& hi, T& xn) {
-    hi = c.bKh * psi_old + c.ah * (pp - pc) * inv_d;
-    lo = c.bKh1 * psi_old1 + c.ah1 * (pc - pm) * inv_d;
-    const T dpsi = (hi - lo) * inv_d;
-    xn = c.bK1 * xi_old + c.a1 * (d2 + dpsi);
-    return dpsi + xn;
+struct Pair {
+    T t, lo, hi, xn;
+};
+
+template <typename T>
+__device__ __forceinline__ Pair<T> cpml_pair(const LayerCoef<T>& c, T psi_old, T psi_old1,
+                                             T xi_old, T pm, T pc, T pp, T d2, T inv_d) {
+    Pair<T> r;
+    r.hi = c.bKh * psi_old + c.ah * (pp - pc) * inv_d;
+    r.lo = c.bKh1 * psi_old1 + c.ah1 * (pc - pm) * inv_d;
+    const T dpsi = (r.hi - r.lo) * inv_d;
+    r.xn = c.bK1 * xi_old + c.a1 * (d2 + dpsi);
+    r.t = dpsi + r.xn;
+    return r;
 }
 
 // Step t+1 at one node from its loads.  ZS: z slab side of the plane (-1:
@@ -82,24 +108,35 @@ __device__ __forceinline__ T cpml_pair(const LayerCoef<T>& c, T psi_old, T psi_o
 // time (that would place the array in scratch memory).
 template <typename T, typename Off, int SA, bool SB, int ZS>
 __device__ __forceinline__ Mid<T> two_first(const March<T, Off, 3>& M, int mz,
-                                            const NodeLoads<T>& q) {
+                                            const NodeLoads<T>& q, T pm, T pc, T xm, T xp) {
     Mid<T> r = {};
     const T two = T(2);
-    const T pc = q.pc, f = q.f;
-    const T d2a = (q.xp - two * pc + q.xm) * M.i2a;
+    const T f = q.f;
+    const T d2a = (xp - two * pc + xm) * M.i2a;
     const T d2b = (q.yp - two * pc + q.ym) * M.i2b;
-    const T d2m = (q.pp - two * pc + q.pm) * M.i2m;
+    const T d2m = (q.pp - two * pc + pm) * M.i2m;
     T damp = T(0);
-    if (SA == 2 && M.sa.in)
-        damp += f * cpml_pair(M.sa.c, q.psa0, q.psa1, q.xia, q.xm, pc, q.xp, d2a, M.ia, r.xl, r.xh,
-                              r.xx);
-    if (SB)
-        damp += f * cpml_pair(M.sb.c, q.psb0, q.psb1, q.xib, q.ym, pc, q.yp, d2b, M.ib, r.yl, r.yh,
-                              r.yx);
+    if (SA == 2 && M.sa.in) {
+        const Pair<T> e = cpml_pair(M.sa.c, q.psa0, q.psa1, q.xia, xm, pc, xp, d2a, M.ia);
+        damp += f * e.t;
+        r.xl = e.lo;
+        r.xh = e.hi;
+        r.xx = e.xn;
+    }
+    if (SB) {
+        const Pair<T> e = cpml_pair(M.sb.c, q.psb0, q.psb1, q.xib, q.ym, pc, q.yp, d2b, M.ib);
+        damp += f * e.t;
+        r.yl = e.lo;
+        r.yh = e.hi;
+        r.yx = e.xn;
+    }
     if (ZS >= 0) {
         const LayerCoef<T> cz = lds_coef(M.lds, M.h1, ZS, mz);
-        damp += f * cpml_pair(cz, q.psm0, q.psm1, q.xim, q.pm, pc, q.pp, d2m, M.imm, r.zl, r.zh,
-                              r.zx);
+        const Pair<T> e = cpml_pair(cz, q.psm0, q.psm1, q.xim, pm, pc, q.pp, d2m, M.imm);
+        damp += f * e.t;
+        r.zl = e.lo;
+        r.zh = e.hi;
+        r.zx = e.xn;
     }
     const T lap = d2a + d2b + d2m;  // (d2x + d2y) + d2z
     r.p1 = two * pc - q.po + f * lap + damp;
@@ -131,62 +168,104 @@ __device__ __forceinline__ void two_second(const March<T, Off, 3>& M, const TwoO
     const T d2m = (pp - two * pc + pm) * M.i2m;
     T damp = T(0);
     if (SA == 2 && M.sa.in) {
-        T lo, hi, xn;
-        damp += f * cpml_pair(M.sa.c, d.xh, d.xl, d.xx, xm, pc, xp, d2a, M.ia, lo, hi, xn);
+        const Pair<T> e = cpml_pair(M.sa.c, d.xh, d.xl, d.xx, xm, pc, xp, d2a, M.ia);
+        damp += f * e.t;
         if (own) {
-            stg(M.sa.p.psi_wr, M.sa.o_psi + (Off)k * M.sa.st_psi, hi);
-            if (M.sa.m == 1) stg(M.sa.p.psi_wr, M.sa.o_psi1 + (Off)k * M.sa.st_psi, lo);
-            stg(O.xi_x, M.sa.o_xi + (Off)k * M.sa.st_xi, xn);
+            stg(M.sa.p.psi_wr, M.sa.o_psi + (Off)k * M.sa.st_psi, e.hi);
+            if (M.sa.m == 1) stg(M.sa.p.psi_wr, M.sa.o_psi1 + (Off)k * M.sa.st_psi, e.lo);
+            stg(O.xi_x, M.sa.o_xi + (Off)k * M.sa.st_xi, e.xn);
         }
     }
     if (SB) {
-        T lo, hi, xn;
-        damp += f * cpml_pair(M.sb.c, d.yh, d.yl, d.yx, ym, pc, yp, d2b, M.ib, lo, hi, xn);
+        const Pair<T> e = cpml_pair(M.sb.c, d.yh, d.yl, d.yx, ym, pc, yp, d2b, M.ib);
+        damp += f * e.t;
         if (own) {
-            stg(M.sb.p.psi_wr, M.sb.o_psi + (Off)k * M.sb.st_psi, hi);
-            if (M.sb.m == 1) stg(M.sb.p.psi_wr, M.sb.o_psi1 + (Off)k * M.sb.st_psi, lo);
-            stg(O.xi_y, M.sb.o_xi + (Off)k * M.sb.st_xi, xn);
+            stg(M.sb.p.psi_wr, M.sb.o_psi + (Off)k * M.sb.st_psi, e.hi);
+            if (M.sb.m == 1) stg(M.sb.p.psi_wr, M.sb.o_psi1 + (Off)k * M.sb.st_psi, e.lo);
+            stg(O.xi_y, M.sb.o_xi + (Off)k * M.sb.st_xi, e.xn);
         }
     }
     if (ZS >= 0) {
         const LayerCoef<T> cz = lds_coef(M.lds, M.h1, ZS, mz);
-        T lo, hi, xn;
-        damp += f * cpml_pair(cz, d.zh, d.zl, d.zx, pm, pc, pp, d2m, M.imm, lo, hi, xn);
+        const Pair<T> e = cpml_pair(cz, d.zh, d.zl, d.zx, pm, pc, pp, d2m, M.imm);
+        damp += f * e.t;
         if (own) {
             T* pw = M.pm_side[ZS < 0 ? 0 : ZS].psi_wr;
-            stg(pw, M.col + (Off)mz * M.sm, hi);
-            if (mz == 1) stg(pw, M.col + (Off)(mz - 1) * M.sm, lo);
-            stg(O.xi_z[ZS < 0 ? 0 : ZS], M.col + (Off)(mz - 1) * M.sm, xn);
+            stg(pw, M.col + (Off)mz * M.sm, e.hi);
+            if (mz == 1) stg(pw, M.col + (Off)(mz - 1) * M.sm, e.lo);
+            stg(O.xi_z[ZS < 0 ? 0 : ZS], M.col + (Off)(mz - 1) * M.sm, e.xn);
         }
     }
     const T lap = d2a + d2b + d2m;
     if (own) stg_stream(O.p2, M.col + (Off)k * M.sm, two * pc - d.pc + f * lap + damp);
 }
 
-// Step t+1 loads of plane k, dispatched on the plane's z slab side.
-template <typename T, typename Off, int SA, bool SB, bool ZM>
-__device__ __forceinline__ void two_load(const StencilArgs<T>& A, const March<T, Off, 3>& M,
-                                         int k, NodeLoads<T>& q) {
-    if (!ZM) {
-        load_node<T, Off, 3, SA, SB, false>(M, k, 0, q);
-        return;
+// Step t+1 loads of march position k into a ring slot.  The wave's own
+// pcur column is loaded once per plane (pp = pcur at k+1; pm and pc are the
+// two planes before, carried in registers) and the x neighbours are lane
+// shifts of pc, so only the two edge lanes load one: q.xm holds pcur at
+// x-1 on lane 0 and at x+1 on lane 63.  pp is loaded for every position
+// below the last plane (the next position needs it as pc); the other loads
+// only for computed planes (1 <= k <= nz-2).  The slot is returned by
+// value: branches that fill different fields of a slot held in memory get
+// merged into stores through a selected address, which would keep the
+// whole ring out of registers.
+template <typename T, typename Off, int SA, bool SB, bool GEN>
+__device__ __forceinline__ NodeLoads<T> two_load_node(const March<T, Off, 3>& M, int nz, int k,
+                                                      int mside) {
+    NodeLoads<T> q = {};
+    constexpr Off E = (Off)sizeof(T);
+    const Off c = M.col + (Off)k * M.sm;
+    const int lane = (int)threadIdx.x;
+    if (!GEN || k + 1 <= nz - 1) q.pp = ldg(M.pcur, c + M.sm);
+    if (GEN && (k < 1 || k > nz - 2)) return q;
+    if (lane == 0 || lane == kTileX - 1) q.xm = ldg(M.pcur, lane == 0 ? c - E : c + E);
+    q.ym = ldg(M.pcur, c - M.sy);
+    q.yp = ldg(M.pcur, c + M.sy);
+    q.po = ldg_stream(M.pold, c);
+    q.f = ldg(M.fact, c);
+    if (SA == 2) {
+        q.psa0 = ldg(M.sa.p.psi_rd, M.sa.o_psi + (Off)k * M.sa.st_psi);
+        q.psa1 = ldg(M.sa.p.psi_rd, M.sa.o_psi1 + (Off)k * M.sa.st_psi);
+        q.xia = ldg((const T*)M.sa.p.xi, M.sa.o_xi + (Off)k * M.sa.st_xi);
     }
+    if (SB) {
+        q.psb0 = ldg(M.sb.p.psi_rd, M.sb.o_psi + (Off)k * M.sb.st_psi);
+        q.psb1 = ldg(M.sb.p.psi_rd, M.sb.o_psi1 + (Off)k * M.sb.st_psi);
+        q.xib = ldg((const T*)M.sb.p.xi, M.sb.o_xi + (Off)k * M.sb.st_xi);
+    }
+    if (mside >= 0) {  // z slab layer (mside: its side)
+        const int mz = k - (mside ? M.mshift[1] : M.mshift[0]);
+        const T* pr = mside ? M.pm_side[1].psi_rd : M.pm_side[0].psi_rd;
+        const T* xr = mside ? M.pm_side[1].xi : M.pm_side[0].xi;
+        q.psm0 = ldg(pr, M.col + (Off)mz * M.sm);
+        q.psm1 = ldg(pr, M.col + (Off)(mz - 1) * M.sm);
+        q.xim = ldg(xr, M.col + (Off)(mz - 1) * M.sm);
+    }
+    return q;
+}
+
+// ZM = false: no z slab layer in the chunk.  One code path for both sides
+// (a branch per side would be merged into stores through selected
+// addresses, which keeps the slot in scratch memory).
+template <typename T, typename Off, int SA, bool SB, bool ZM, bool GEN>
+__device__ __forceinline__ NodeLoads<T> two_load(const StencilArgs<T>& A,
+                                                 const March<T, Off, 3>& M, int k) {
     int mz;
-    const int zs = classify(k, A.nz, A.halo, A.active[2], &mz);
-    if (zs == 0) load_node<T, Off, 3, SA, SB, true>(M, k, 0, q);
-    else if (zs == 1) load_node<T, Off, 3, SA, SB, true>(M, k, 1, q);
-    else load_node<T, Off, 3, SA, SB, false>(M, k, 0, q);
+    const int zs = ZM ? classify(k, A.nz, A.halo, A.active[2], &mz) : -1;
+    return two_load_node<T, Off, SA, SB, GEN>(M, (int)A.nz, k, zs);
 }
 
 template <typename T, typename Off, int SA, bool SB, bool ZM>
 __device__ __forceinline__ Mid<T> two_first_at(const StencilArgs<T>& A, const March<T, Off, 3>& M,
-                                               int k, const NodeLoads<T>& q) {
-    if (!ZM) return two_first<T, Off, SA, SB, -1>(M, 0, q);
+                                               int k, const NodeLoads<T>& q, T pm, T pc, T xm,
+                                               T xp) {
+    if (!ZM) return two_first<T, Off, SA, SB, -1>(M, 0, q, pm, pc, xm, xp);
     int mz;
     const int zs = classify(k, A.nz, A.halo, A.active[2], &mz);
-    if (zs == 0) return two_first<T, Off, SA, SB, 0>(M, mz, q);
-    if (zs == 1) return two_first<T, Off, SA, SB, 1>(M, mz, q);
-    return two_first<T, Off, SA, SB, -1>(M, mz, q);
+    if (zs == 0) return two_first<T, Off, SA, SB, 0>(M, mz, q, pm, pc, xm, xp);
+    if (zs == 1) return two_first<T, Off, SA, SB, 1>(M, mz, q, pm, pc, xm, xp);
+    return two_first<T, Off, SA, SB, -1>(M, mz, q, pm, pc, xm, xp);
 }
 
 template <typename T, typename Off, int SA, bool SB, bool ZM>
@@ -212,6 +291,7 @@ struct TwoCtx {
     NodeLoads<T> qr[PF];
     Mid<T> d;
     T p1m;
+    T pz0, pz1;   // pcur of the wave's column at planes k-1 and k
     int slot;
 };
 
@@ -220,6 +300,7 @@ struct TwoGeom {
     int wy, kbeg, kend, nz, ksrc_lo, ksrc_hi, s0, s1;
     int64_t col_el, plane;
     bool in1, own;
+    bool inx;     // the lane's x is its clamped x (pcur there is the lane's own column)
 };
 
 // One march position k: step t+1 at plane k, refill of ring slot U with
@@ -233,8 +314,17 @@ __device__ __forceinline__ void two_iter(const Stencil2Args<T>& B, const March<T
     const StencilArgs<T>& A = B.s;
     const int lane = (int)threadIdx.x;
     Mid<T> dn = {};
+    const T pp = C.qr[U].pp;
     if (!GEN || (k >= 1 && k <= G.nz - 2)) {
-        dn = two_first_at<T, Off, SA, SB, ZM>(A, M, k, C.qr[U]);
+        // x neighbours: lane shifts of pc; a lane outside the grid's x range
+        // (clamped) stands for the Dirichlet node there, 0
+        const T pc = C.pz1;
+        const T pcx = G.inx ? pc : T(0);
+        const T sl = lane_shift(pcx, true), sr = lane_shift(pcx, false);
+        const T edge = C.qr[U].xm;
+        const T xm = lane == 0 ? edge : sl;
+        const T xp = lane == kTileX - 1 ? edge : sr;
+        dn = two_first_at<T, Off, SA, SB, ZM>(A, M, k, C.qr[U], C.pz0, pc, xm, xp);
         // inject_sources! of step t+1, in source order
         if (k >= G.ksrc_lo && k <= G.ksrc_hi) {
             const int64_t lin = G.col_el + (int64_t)k * G.plane;
@@ -243,8 +333,10 @@ __device__ __forceinline__ void two_iter(const Stencil2Args<T>& B, const March<T
         }
         dn.p1 = G.in1 ? dn.p1 : T(0);
     }
+    C.pz0 = C.pz1;
+    C.pz1 = pp;
     const int kr = k + PF;
-    if (!GEN || (kr <= G.kend && kr <= G.nz - 2)) two_load<T, Off, SA, SB, ZM>(A, M, kr, C.qr[U]);
+    if (!GEN || kr <= G.kend) C.qr[U] = two_load<T, Off, SA, SB, ZM, GEN>(A, M, kr);
     __builtin_amdgcn_sched_barrier(0);
     ring[C.slot][G.wy][lane] = dn.p1;
     if (G.own && (!GEN || (k >= G.kbeg && k < G.kend)))
@@ -252,8 +344,8 @@ __device__ __forceinline__ void two_iter(const Stencil2Args<T>& B, const March<T
     __syncthreads();
     if (!GEN || k - 1 >= G.kbeg) {
         const int sp = C.slot == 0 ? 2 : C.slot - 1;  // plane k-1
-        const T xm = __shfl_up(C.d.p1, 1u, kTileX);
-        const T xp = __shfl_down(C.d.p1, 1u, kTileX);
+        const T xm = lane_shift(C.d.p1, true);
+        const T xp = lane_shift(C.d.p1, false);
         T ym = T(0), yp = T(0);
         if (G.wy >= 1 && G.wy <= W - 2) {
             ym = ring[sp][G.wy - 1][lane];
@@ -292,11 +384,12 @@ __device__ __forceinline__ void two_march(const Stencil2Args<T>& B, const March<
                                           const TwoGeom& G) {
     const StencilArgs<T>& A = B.s;
     TwoCtx<T, PF> C;
+    C.pz0 = G.kbeg >= 2 ? ldg(M.pcur, M.col + (Off)(G.kbeg - 2) * M.sm) : T(0);
+    C.pz1 = ldg(M.pcur, M.col + (Off)(G.kbeg - 1) * M.sm);
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
-        C.qr[u] = NodeLoads<T>{};
         const int kk = G.kbeg - 1 + u;
-        if (kk >= 1 && kk <= G.kend && kk <= G.nz - 2) two_load<T, Off, SA, SB, ZM>(A, M, kk, C.qr[u]);
+        C.qr[u] = kk <= G.kend ? two_load<T, Off, SA, SB, ZM, true>(A, M, kk) : NodeLoads<T>{};
         __builtin_amdgcn_sched_barrier(0);
     }
     C.d = Mid<T>{};
@@ -407,6 +500,7 @@ __global__ __launch_bounds__(64 * W) void stencil3d_two(const Stencil2Args<T> B)
     G.nz = (int)nz;
     G.in1 = in1;
     G.own = own;
+    G.inx = i == ic;
 
     const bool xs = __any(M.sa.in);
     const bool ys = M.sb.in;  // wave-uniform
