@@ -108,12 +108,13 @@ __device__ __forceinline__ Pair<T> cpml_pair(const LayerCoef<T>& c, T psi_old, T
 // time (that would place the array in scratch memory).
 template <typename T, typename Off, int SA, bool SB, int ZS>
 __device__ __forceinline__ Mid<T> two_first(const March<T, Off, 3>& M, int mz,
-                                            const NodeLoads<T>& q, T pm, T pc, T xm, T xp) {
+                                            const NodeLoads<T>& q, T pm, T pc, T xm, T xp, T ym,
+                                            T yp) {
     Mid<T> r = {};
     const T two = T(2);
     const T f = q.f;
     const T d2a = (xp - two * pc + xm) * M.i2a;
-    const T d2b = (q.yp - two * pc + q.ym) * M.i2b;
+    const T d2b = (yp - two * pc + ym) * M.i2b;
     const T d2m = (q.pp - two * pc + pm) * M.i2m;
     T damp = T(0);
     if (SA == 2 && M.sa.in) {
@@ -124,7 +125,7 @@ __device__ __forceinline__ Mid<T> two_first(const March<T, Off, 3>& M, int mz,
         r.xx = e.xn;
     }
     if (SB) {
-        const Pair<T> e = cpml_pair(M.sb.c, q.psb0, q.psb1, q.xib, q.ym, pc, q.yp, d2b, M.ib);
+        const Pair<T> e = cpml_pair(M.sb.c, q.psb0, q.psb1, q.xib, ym, pc, yp, d2b, M.ib);
         damp += f * e.t;
         r.yl = e.lo;
         r.yh = e.hi;
@@ -202,17 +203,20 @@ __device__ __forceinline__ void two_second(const March<T, Off, 3>& M, const TwoO
 
 // Step t+1 loads of march position k into a ring slot.  The wave's own
 // pcur column is loaded once per plane (pp = pcur at k+1; pm and pc are the
-// two planes before, carried in registers) and the x neighbours are lane
-// shifts of pc, so only the two edge lanes load one: q.xm holds pcur at
-// x-1 on lane 0 and at x+1 on lane 63.  pp is loaded for every position
+// two planes before, carried in registers); the x neighbours are lane
+// shifts of pc, so only the two edge lanes load one (q.xm: pcur at x-1 on
+// lane 0, at x+1 on lane 63); the y neighbours come from the other waves'
+// columns through LDS, so only the block's outer rows load one (q.ym: pcur
+// at y-1 on row 0, at y+1 on row W-1; one load with a selected offset, as
+// two exclusive loads would again be merged through a selected address).  pp is loaded for every position
 // below the last plane (the next position needs it as pc); the other loads
 // only for computed planes (1 <= k <= nz-2).  The slot is returned by
 // value: branches that fill different fields of a slot held in memory get
 // merged into stores through a selected address, which would keep the
 // whole ring out of registers.
-template <typename T, typename Off, int SA, bool SB, bool GEN>
+template <typename T, typename Off, int W, int SA, bool SB, bool GEN>
 __device__ __forceinline__ NodeLoads<T> two_load_node(const March<T, Off, 3>& M, int nz, int k,
-                                                      int mside) {
+                                                      int mside, int wy) {
     NodeLoads<T> q = {};
     constexpr Off E = (Off)sizeof(T);
     const Off c = M.col + (Off)k * M.sm;
@@ -220,8 +224,7 @@ __device__ __forceinline__ NodeLoads<T> two_load_node(const March<T, Off, 3>& M,
     if (!GEN || k + 1 <= nz - 1) q.pp = ldg(M.pcur, c + M.sm);
     if (GEN && (k < 1 || k > nz - 2)) return q;
     if (lane == 0 || lane == kTileX - 1) q.xm = ldg(M.pcur, lane == 0 ? c - E : c + E);
-    q.ym = ldg(M.pcur, c - M.sy);
-    q.yp = ldg(M.pcur, c + M.sy);
+    if (wy == 0 || wy == W - 1) q.ym = ldg(M.pcur, wy == 0 ? c - M.sy : c + M.sy);
     q.po = ldg_stream(M.pold, c);
     q.f = ldg(M.fact, c);
     if (SA == 2) {
@@ -248,24 +251,24 @@ __device__ __forceinline__ NodeLoads<T> two_load_node(const March<T, Off, 3>& M,
 // ZM = false: no z slab layer in the chunk.  One code path for both sides
 // (a branch per side would be merged into stores through selected
 // addresses, which keeps the slot in scratch memory).
-template <typename T, typename Off, int SA, bool SB, bool ZM, bool GEN>
+template <typename T, typename Off, int W, int SA, bool SB, bool ZM, bool GEN>
 __device__ __forceinline__ NodeLoads<T> two_load(const StencilArgs<T>& A,
-                                                 const March<T, Off, 3>& M, int k) {
+                                                 const March<T, Off, 3>& M, int k, int wy) {
     int mz;
     const int zs = ZM ? classify(k, A.nz, A.halo, A.active[2], &mz) : -1;
-    return two_load_node<T, Off, SA, SB, GEN>(M, (int)A.nz, k, zs);
+    return two_load_node<T, Off, W, SA, SB, GEN>(M, (int)A.nz, k, zs, wy);
 }
 
 template <typename T, typename Off, int SA, bool SB, bool ZM>
 __device__ __forceinline__ Mid<T> two_first_at(const StencilArgs<T>& A, const March<T, Off, 3>& M,
                                                int k, const NodeLoads<T>& q, T pm, T pc, T xm,
-                                               T xp) {
-    if (!ZM) return two_first<T, Off, SA, SB, -1>(M, 0, q, pm, pc, xm, xp);
+                                               T xp, T ym, T yp) {
+    if (!ZM) return two_first<T, Off, SA, SB, -1>(M, 0, q, pm, pc, xm, xp, ym, yp);
     int mz;
     const int zs = classify(k, A.nz, A.halo, A.active[2], &mz);
-    if (zs == 0) return two_first<T, Off, SA, SB, 0>(M, mz, q, pm, pc, xm, xp);
-    if (zs == 1) return two_first<T, Off, SA, SB, 1>(M, mz, q, pm, pc, xm, xp);
-    return two_first<T, Off, SA, SB, -1>(M, mz, q, pm, pc, xm, xp);
+    if (zs == 0) return two_first<T, Off, SA, SB, 0>(M, mz, q, pm, pc, xm, xp, ym, yp);
+    if (zs == 1) return two_first<T, Off, SA, SB, 1>(M, mz, q, pm, pc, xm, xp, ym, yp);
+    return two_first<T, Off, SA, SB, -1>(M, mz, q, pm, pc, xm, xp, ym, yp);
 }
 
 template <typename T, typename Off, int SA, bool SB, bool ZM>
@@ -301,6 +304,7 @@ struct TwoGeom {
     int64_t col_el, plane;
     bool in1, own;
     bool inx;     // the lane's x is its clamped x (pcur there is the lane's own column)
+    bool iny;     // the wave's y is its clamped y
 };
 
 // One march position k: step t+1 at plane k, refill of ring slot U with
@@ -310,7 +314,8 @@ struct TwoGeom {
 template <typename T, typename Off, int W, int PF, int SA, bool SB, bool ZM, bool GEN, int U>
 __device__ __forceinline__ void two_iter(const Stencil2Args<T>& B, const March<T, Off, 3>& M,
                                          const TwoOut<T, Off>& O, T (*ring)[W][kTileX],
-                                         const TwoGeom& G, TwoCtx<T, PF>& C, int k) {
+                                         T (*pring)[W][kTileX], const TwoGeom& G,
+                                         TwoCtx<T, PF>& C, int k) {
     const StencilArgs<T>& A = B.s;
     const int lane = (int)threadIdx.x;
     Mid<T> dn = {};
@@ -324,7 +329,9 @@ __device__ __forceinline__ void two_iter(const Stencil2Args<T>& B, const March<T
         const T edge = C.qr[U].xm;
         const T xm = lane == 0 ? edge : sl;
         const T xp = lane == kTileX - 1 ? edge : sr;
-        dn = two_first_at<T, Off, SA, SB, ZM>(A, M, k, C.qr[U], C.pz0, pc, xm, xp);
+        const T ym = G.wy == 0 ? C.qr[U].ym : pring[C.slot][G.wy - 1 < 0 ? 0 : G.wy - 1][lane];
+        const T yp = G.wy == W - 1 ? C.qr[U].ym : pring[C.slot][G.wy + 1 > W - 1 ? W - 1 : G.wy + 1][lane];
+        dn = two_first_at<T, Off, SA, SB, ZM>(A, M, k, C.qr[U], C.pz0, pc, xm, xp, ym, yp);
         // inject_sources! of step t+1, in source order
         if (k >= G.ksrc_lo && k <= G.ksrc_hi) {
             const int64_t lin = G.col_el + (int64_t)k * G.plane;
@@ -336,8 +343,11 @@ __device__ __forceinline__ void two_iter(const Stencil2Args<T>& B, const March<T
     C.pz0 = C.pz1;
     C.pz1 = pp;
     const int kr = k + PF;
-    if (!GEN || kr <= G.kend) C.qr[U] = two_load<T, Off, SA, SB, ZM, GEN>(A, M, kr);
+    if (!GEN || kr <= G.kend) C.qr[U] = two_load<T, Off, W, SA, SB, ZM, GEN>(A, M, kr, G.wy);
     __builtin_amdgcn_sched_barrier(0);
+    // this column's pcur at plane k+1 for the neighbouring rows (0 for a
+    // Dirichlet row), and p^{t+1} at plane k
+    pring[C.slot == 2 ? 0 : C.slot + 1][G.wy][lane] = G.iny ? pp : T(0);
     ring[C.slot][G.wy][lane] = dn.p1;
     if (G.own && (!GEN || (k >= G.kbeg && k < G.kend)))
         stg_stream(O.p1, M.col + (Off)k * M.sm, dn.p1);
@@ -364,10 +374,12 @@ template <typename T, typename Off, int W, int PF, int SA, bool SB, bool ZM, boo
           int... Us>
 __device__ __forceinline__ void two_group(const Stencil2Args<T>& B, const March<T, Off, 3>& M,
                                           const TwoOut<T, Off>& O, T (*ring)[W][kTileX],
-                                          const TwoGeom& G, TwoCtx<T, PF>& C, int k,
+                                          T (*pring)[W][kTileX], const TwoGeom& G,
+                                          TwoCtx<T, PF>& C, int k,
                                           std::integer_sequence<int, Us...>) {
     ((!CHECK || k + Us <= G.kend
-          ? (two_iter<T, Off, W, PF, SA, SB, ZM, GEN, Us % PF>(B, M, O, ring, G, C, k + Us), 0)
+          ? (two_iter<T, Off, W, PF, SA, SB, ZM, GEN, Us % PF>(B, M, O, ring, pring, G, C, k + Us),
+             0)
           : 0),
      ...);
 }
@@ -381,7 +393,7 @@ __device__ __forceinline__ void two_group(const Stencil2Args<T>& B, const March<
 template <typename T, typename Off, int W, int PF, int SA, bool SB, bool ZM>
 __device__ __forceinline__ void two_march(const Stencil2Args<T>& B, const March<T, Off, 3>& M,
                                           const TwoOut<T, Off>& O, T (*ring)[W][kTileX],
-                                          const TwoGeom& G) {
+                                          T (*pring)[W][kTileX], const TwoGeom& G) {
     const StencilArgs<T>& A = B.s;
     TwoCtx<T, PF> C;
     C.pz0 = G.kbeg >= 2 ? ldg(M.pcur, M.col + (Off)(G.kbeg - 2) * M.sm) : T(0);
@@ -389,29 +401,32 @@ __device__ __forceinline__ void two_march(const Stencil2Args<T>& B, const March<
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
         const int kk = G.kbeg - 1 + u;
-        C.qr[u] = kk <= G.kend ? two_load<T, Off, SA, SB, ZM, true>(A, M, kk) : NodeLoads<T>{};
+        C.qr[u] = kk <= G.kend ? two_load<T, Off, W, SA, SB, ZM, true>(A, M, kk, G.wy) : NodeLoads<T>{};
         __builtin_amdgcn_sched_barrier(0);
     }
     C.d = Mid<T>{};
     C.p1m = T(0);
     C.slot = (G.kbeg - 1) % 3;
+    pring[C.slot][G.wy][threadIdx.x] = G.iny ? C.pz1 : T(0);
+    __syncthreads();
     constexpr int H = (2 + PF - 1) / PF * PF;
     int k = G.kbeg - 1;
-    two_group<T, Off, W, PF, SA, SB, ZM, true, true>(B, M, O, ring, G, C, k,
+    two_group<T, Off, W, PF, SA, SB, ZM, true, true>(B, M, O, ring, pring, G, C, k,
                                                      std::make_integer_sequence<int, H>{});
     k += H;
     const int last = (G.kend < G.nz - 2 ? G.kend : G.nz - 2) - PF;  // last steady position
     for (; k + PF - 1 <= last; k += PF)
-        two_group<T, Off, W, PF, SA, SB, ZM, false, false>(B, M, O, ring, G, C, k,
+        two_group<T, Off, W, PF, SA, SB, ZM, false, false>(B, M, O, ring, pring, G, C, k,
                                                            std::make_integer_sequence<int, PF>{});
     for (; k <= G.kend; k += PF)
-        two_group<T, Off, W, PF, SA, SB, ZM, true, true>(B, M, O, ring, G, C, k,
+        two_group<T, Off, W, PF, SA, SB, ZM, true, true>(B, M, O, ring, pring, G, C, k,
                                                          std::make_integer_sequence<int, PF>{});
 }
 
 template <typename T, typename Off, int W, int PF>
 __global__ __launch_bounds__(64 * W) void stencil3d_two(const Stencil2Args<T> B) {
-    __shared__ T ring[3][W][kTileX];
+    __shared__ T ring[3][W][kTileX];    // p^{t+1} planes
+    __shared__ T pring[3][W][kTileX];   // pcur planes
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* lds = reinterpret_cast<T*>(smem_raw);
     const StencilArgs<T>& A = B.s;
@@ -501,13 +516,14 @@ __global__ __launch_bounds__(64 * W) void stencil3d_two(const Stencil2Args<T> B)
     G.in1 = in1;
     G.own = own;
     G.inx = i == ic;
+    G.iny = j == jc;
 
     const bool xs = __any(M.sa.in);
     const bool ys = M.sb.in;  // wave-uniform
     // any z slab layer among the planes this chunk reads (block-uniform)
     const bool zm = (A.active[2][0] && kbeg - 1 <= halo) || (A.active[2][1] && kend >= nz - halo - 1);
 #define AWCPML_TWO(SA_, SB_, ZM_)                                                             \
-    two_march<T, Off, W, PF, SA_, SB_, ZM_>(B, M, O, ring, G)
+    two_march<T, Off, W, PF, SA_, SB_, ZM_>(B, M, O, ring, pring, G)
     if (zm) {
         if (xs) {
             if (ys) AWCPML_TWO(2, true, true);
