@@ -43,9 +43,13 @@ def two_step_candidates(plan):
     if plan.ndim != 3:
         return []
     n_march = plan.n[2] - 2
-    rows = (16, 8) if plan.dtype != 0 else (8, 4)
-    lens = sorted({min(n_march, v) for v in (24, 48, 96)})
-    return [(w, ml, pf) for w in rows for ml in lens for pf in (1, 2)]
+    out = []
+    for w in ((16, 8) if plan.dtype != 0 else (8, 4)):
+        lens = (24, 48, 96, 144) if w == 16 else (24, 48, 96)
+        pfs = (1, 2, 3) if w == 16 else (1, 2)
+        for ml in sorted({min(n_march, v) for v in lens}):
+            out += [(w, ml, pf) for pf in pfs]
+    return out
 
 
 def _apply(plan, best):
