@@ -10,9 +10,9 @@ reset afterwards.
     best = autotune(plan)            # {"block_y": 4, "march_len": 148, ...}
 
 Multi-shot plans also try shot groups (the waves of a block march the same
-rows of several shots, sharing the model rows they read).  3D plans also
-time two-step launches (``Plan.set_time_block``: two time steps per launch,
-bit-identical results) per time step, and keep them when they are faster.
+rows of several shots, sharing the model rows they read).  Two-step
+launches (``Plan.set_time_block``: two time steps per launch, bit-identical
+results) are timed per time step too, and kept when they are faster.
 """
 import json
 import os
@@ -40,8 +40,10 @@ def candidates(plan):
 
 def two_step_candidates(plan):
     """(block_rows, march_len, prefetch) of two-step launches worth timing."""
-    if plan.ndim != 3:
-        return []
+    if plan.ndim == 2:
+        n_march = plan.n[1] - 2
+        return [(w, min(n_march, ml), pf) for w in (4, 8) for ml in (32, 64, 128)
+                for pf in (1, 2, 3)]
     n_march = plan.n[2] - 2
     out = []
     for w in ((16, 8) if plan.dtype != 0 else (8, 4)):
@@ -58,7 +60,7 @@ def _apply(plan, best):
     tb = best.get("time_block")
     if tb:
         plan.set_time_block(2, tb["block_rows"], tb["march_len"], tb["prefetch"])
-    elif plan.ndim == 3:
+    else:
         plan.set_time_block(1)
 
 
@@ -83,8 +85,7 @@ def autotune(plan, cands=None, launches=8, cache=DEFAULT_CACHE, use_cache=True):
         best = db[key]
         _apply(plan, best)
         return dict(best, cached=True)
-    if plan.ndim == 3:
-        plan.set_time_block(1)
+    plan.set_time_block(1)
     results = []
     for cand in (cands or candidates(plan)):
         by, ml, pf = cand[:3]

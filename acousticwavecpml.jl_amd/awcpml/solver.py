@@ -121,8 +121,8 @@ class Plan:
         return g.value
 
     def set_time_block(self, steps, block_rows=0, march_len=0, prefetch=0):
-        """steps = 2: two time steps per stencil launch where that applies
-        (3D plans); results are bit-identical to one step per launch."""
+        """steps = 2: two time steps per stencil launch where that applies;
+        results are bit-identical to one step per launch."""
         _lib.check(_lib.load().awcpml_set_time_block(self._h, int(steps), int(block_rows),
                                                      int(march_len), int(prefetch)))
 

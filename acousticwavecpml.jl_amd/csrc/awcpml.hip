@@ -452,18 +452,20 @@ int do_step(awcpml_plan* P, int64_t it0) {
     return AWCPML_OK;
 }
 
-// Two-step launches apply: requested, 3D, every field buffer zero on the
+// Two-step launches apply: requested, every field buffer zero on the
 // Dirichlet nodes, 32-bit byte offsets, no d(pnew)/d(fact) output.
 bool two_ok(const awcpml_plan* P) {
-    return P->time_block == 2 && P->ndim == 3 && P->p_spare && !P->q_cur && !P->src_on_boundary &&
+    return P->time_block == 2 && P->p_spare && !P->q_cur && !P->src_on_boundary &&
            !P->boundary_dirty && (uint64_t)P->field_elems * P->esize < (1ull << 32);
 }
 
 void two_tiling(const awcpml_plan* P, int* rows, int* march, int* prefetch) {
     int W = P->tb_rows;
-    if (W <= 0) W = P->dtype == AWCPML_F64 ? 8 : 16;
+    if (W <= 0) W = P->ndim == 2 ? 4 : (P->dtype == AWCPML_F64 ? 8 : 16);
     int ml = P->tb_march;
-    if (ml <= 0) {
+    if (ml <= 0 && P->ndim == 2) {
+        ml = (int)std::max<int64_t>(8, std::min<int64_t>(64, (P->n[1] - 2) / 16));
+    } else if (ml <= 0) {
         // enough blocks to fill 256 CUs several times over
         const int64_t tiles = ((P->n[0] - 2 + awcpml::kOwnX - 1) / awcpml::kOwnX) *
                               ((P->n[1] - 2 + W - 3) / (W - 2)) * P->nshots;
@@ -494,10 +496,17 @@ awcpml::Stencil2Args<T> make_args2(const awcpml_plan* P, int64_t it0, int W, int
     A.shot_group = 1;
     A.nsgroups = P->nshots;
     A.tiles_x = (int)((P->n[0] - 2 + awcpml::kOwnX - 1) / awcpml::kOwnX);
-    A.tiles_y = (int)((P->n[1] - 2 + W - 3) / (W - 2));
     A.march_len = ml;
-    A.nchunks = (int)((P->n[2] - 2 + ml - 1) / ml);
-    A.logical_blocks = A.tiles_x * A.tiles_y * A.nchunks * P->nshots;
+    if (P->ndim == 3) {
+        A.tiles_y = (int)((P->n[1] - 2 + W - 3) / (W - 2));
+        A.nchunks = (int)((P->n[2] - 2 + ml - 1) / ml);
+        A.logical_blocks = A.tiles_x * A.tiles_y * A.nchunks * P->nshots;
+    } else {
+        // a block's W waves march W consecutive chunks of rows
+        A.nchunks = (int)((P->n[1] - 2 + ml - 1) / ml);
+        A.tiles_y = (A.nchunks + W - 1) / W;
+        A.logical_blocks = A.tiles_x * A.tiles_y * P->nshots;
+    }
     A.grid_blocks = (int)round_up(A.logical_blocks, awcpml::kNumXcd);
     B.p1 = (T*)P->p[(P->rot + 2) % 3];
     B.p2 = (T*)P->p_spare;
@@ -519,7 +528,8 @@ int launch_stencil2_t(awcpml_plan* P, int64_t it0) {
     L.block = dim3(awcpml::kTileX, W);
     L.lds = (size_t)8 * (P->halo + 1) * sizeof(T);
     L.stream = P->stream;
-    awcpml::launch_plan_stencil2<T>(B, L, W, pf);
+    if (P->ndim == 3) awcpml::launch_plan_stencil2<T, 3>(B, L, W, pf);
+    else awcpml::launch_plan_stencil2<T, 2>(B, L, W, pf);
     HIP_TRY(hipGetLastError());
     return AWCPML_OK;
 }
@@ -1240,10 +1250,10 @@ int awcpml_set_time_block(awcpml_plan* P, int steps, int block_rows, int march_l
     int st = check_plan(P);
     if (st) return st;
     if (steps != 1 && steps != 2) return fail(AWCPML_ERR_ARG, "steps must be 1 or 2 (got %d)", steps);
-    if (steps == 2 && P->ndim != 3)
-        return fail(AWCPML_ERR_ARG, "two steps per launch are implemented for 3D plans");
-    if (block_rows != 0 && block_rows != 4 && block_rows != 8 && block_rows != 16)
-        return fail(AWCPML_ERR_ARG, "block_rows must be 0, 4, 8 or 16 (got %d)", block_rows);
+    if (P->ndim == 3 && block_rows != 0 && block_rows != 4 && block_rows != 8 && block_rows != 16)
+        return fail(AWCPML_ERR_ARG, "block_rows must be 0, 4, 8 or 16 for 3D plans (got %d)", block_rows);
+    if (P->ndim == 2 && block_rows != 0 && block_rows != 4 && block_rows != 8)
+        return fail(AWCPML_ERR_ARG, "block_rows must be 0, 4 or 8 for 2D plans (got %d)", block_rows);
     if (march_len < 0) return fail(AWCPML_ERR_ARG, "march_len < 0");
     if (prefetch < 0 || prefetch > 3) return fail(AWCPML_ERR_ARG, "prefetch must be 0..3 (got %d)", prefetch);
     if (steps == 2 && !P->p_spare) {

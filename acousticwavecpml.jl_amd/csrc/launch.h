@@ -25,9 +25,10 @@ void launch_plan_stencil(const StencilArgs<T>& A, const LaunchShape& L, int pipe
 template <typename T, int NDIM>
 void launch_batch_stencil(const BatchArgs<T>& A, const LaunchShape& L, int pipe, bool off64);
 
-// Two time steps per launch (3D plans); rows: waves per block (4, 8, 16);
-// pf: planes of loads in flight per wave (1..3).
-template <typename T>
+// Two time steps per launch; rows: waves per block (3D: 4, 8, 16 rows of a
+// tile; 2D: 4 or 8 march chunks); pf: march positions of loads in flight
+// per wave (1..3).
+template <typename T, int NDIM>
 void launch_plan_stencil2(const Stencil2Args<T>& B, const LaunchShape& L, int rows, int pf);
 
 }  // namespace awcpml

@@ -33,27 +33,40 @@ void launch_plan_stencil(const StencilArgs<T>& A, const LaunchShape& L, int pipe
 }
 template void launch_plan_stencil<AWCPML_REAL, AWCPML_NDIM>(const StencilArgs<AWCPML_REAL>&,
                                                            const LaunchShape&, int, bool);
-#if AWCPML_NDIM == 3
 template <typename T, int W>
 static void go_two(const Stencil2Args<T>& B, const LaunchShape& L, int pf) {
+#if AWCPML_NDIM == 3
     switch (pf) {
         case 1: stencil3d_two<T, uint32_t, W, 1><<<L.grid, L.block, L.lds, L.stream>>>(B); break;
         case 3: stencil3d_two<T, uint32_t, W, 3><<<L.grid, L.block, L.lds, L.stream>>>(B); break;
         default: stencil3d_two<T, uint32_t, W, 2><<<L.grid, L.block, L.lds, L.stream>>>(B); break;
     }
+#else
+    switch (pf) {
+        case 1: stencil2d_two<T, uint32_t, W, 1><<<L.grid, L.block, L.lds, L.stream>>>(B); break;
+        case 3: stencil2d_two<T, uint32_t, W, 3><<<L.grid, L.block, L.lds, L.stream>>>(B); break;
+        default: stencil2d_two<T, uint32_t, W, 2><<<L.grid, L.block, L.lds, L.stream>>>(B); break;
+    }
+#endif
 }
 
-template <typename T>
+template <typename T, int NDIM>
 void launch_plan_stencil2(const Stencil2Args<T>& B, const LaunchShape& L, int rows, int pf) {
+#if AWCPML_NDIM == 3
     switch (rows) {
         case 4: go_two<T, 4>(B, L, pf); break;
         case 16: go_two<T, 16>(B, L, pf); break;
         default: go_two<T, 8>(B, L, pf); break;
     }
-}
-template void launch_plan_stencil2<AWCPML_REAL>(const Stencil2Args<AWCPML_REAL>&, const LaunchShape&,
-                                                int, int);
+#else
+    switch (rows) {
+        case 8: go_two<T, 8>(B, L, pf); break;
+        default: go_two<T, 4>(B, L, pf); break;
+    }
 #endif
+}
+template void launch_plan_stencil2<AWCPML_REAL, AWCPML_NDIM>(const Stencil2Args<AWCPML_REAL>&,
+                                                            const LaunchShape&, int, int);
 #else
 template <typename T, typename Off, int P, int NDIM>
 static void go_batch(const BatchArgs<T>& A, const LaunchShape& L) {

@@ -544,4 +544,325 @@ __global__ __launch_bounds__(64 * W) void stencil3d_two(const Stencil2Args<T> B)
 #undef AWCPML_TWO
 }
 
+// ---------------------------------------------------------------------
+// 2D: two time steps per launch.  Block (64, BY): lane -> x (62 owned
+// lanes, as in 3D), wave -> one chunk of rows along y, the march axis.
+// Step t+2 at row k-1 needs p^{t+1} at rows k-2..k of the wave's own
+// column (registers) and at x-1, x+1 (DPP lane shifts), so the waves of a
+// block are independent: no LDS exchange and no barrier.
+// ---------------------------------------------------------------------
+
+// Step t+1 at one node (2D).  ZS: y slab side of the row (-1: none).
+template <typename T, typename Off, int SA, int ZS>
+__device__ __forceinline__ Mid<T> two2_first(const March<T, Off, 2>& M, int mz,
+                                             const NodeLoads<T>& q, T pm, T pc, T xm, T xp) {
+    Mid<T> r = {};
+    const T two = T(2);
+    const T f = q.f;
+    const T d2a = (xp - two * pc + xm) * M.i2a;
+    const T d2m = (q.pp - two * pc + pm) * M.i2m;
+    T damp = T(0);
+    if (SA == 2 && M.sa.in) {
+        const Pair<T> e = cpml_pair(M.sa.c, q.psa0, q.psa1, q.xia, xm, pc, xp, d2a, M.ia);
+        damp += f * e.t;
+        r.xl = e.lo;
+        r.xh = e.hi;
+        r.xx = e.xn;
+    }
+    if (ZS >= 0) {
+        const LayerCoef<T> cz = lds_coef(M.lds, M.h1, ZS, mz);
+        const Pair<T> e = cpml_pair(cz, q.psm0, q.psm1, q.xim, pm, pc, q.pp, d2m, M.imm);
+        damp += f * e.t;
+        r.zl = e.lo;
+        r.zh = e.hi;
+        r.zx = e.xn;
+    }
+    const T lap = d2a + d2m;  // d2x + d2y
+    r.p1 = two * pc - q.po + f * lap + damp;
+    r.pc = pc;
+    r.f = f;
+    return r;
+}
+
+// Step t+2 at one node (2D, row k); stores on owned nodes only.
+template <typename T, typename Off, int SA, int ZS>
+__device__ __forceinline__ void two2_second(const March<T, Off, 2>& M, const TwoOut<T, Off>& O,
+                                            const Mid<T>& d, T pm, T pp, T xm, T xp, bool own,
+                                            int mz, int k) {
+    const T two = T(2);
+    const T pc = d.p1, f = d.f;
+    const T d2a = (xp - two * pc + xm) * M.i2a;
+    const T d2m = (pp - two * pc + pm) * M.i2m;
+    T damp = T(0);
+    if (SA == 2 && M.sa.in) {
+        const Pair<T> e = cpml_pair(M.sa.c, d.xh, d.xl, d.xx, xm, pc, xp, d2a, M.ia);
+        damp += f * e.t;
+        if (own) {
+            stg(M.sa.p.psi_wr, M.sa.o_psi + (Off)k * M.sa.st_psi, e.hi);
+            if (M.sa.m == 1) stg(M.sa.p.psi_wr, M.sa.o_psi1 + (Off)k * M.sa.st_psi, e.lo);
+            stg(O.xi_x, M.sa.o_xi + (Off)k * M.sa.st_xi, e.xn);
+        }
+    }
+    if (ZS >= 0) {
+        const LayerCoef<T> cz = lds_coef(M.lds, M.h1, ZS, mz);
+        const Pair<T> e = cpml_pair(cz, d.zh, d.zl, d.zx, pm, pc, pp, d2m, M.imm);
+        damp += f * e.t;
+        if (own) {
+            T* pw = M.pm_side[ZS < 0 ? 0 : ZS].psi_wr;
+            stg(pw, M.col + (Off)mz * M.sm, e.hi);
+            if (mz == 1) stg(pw, M.col + (Off)(mz - 1) * M.sm, e.lo);
+            stg(O.xi_z[ZS < 0 ? 0 : ZS], M.col + (Off)(mz - 1) * M.sm, e.xn);
+        }
+    }
+    const T lap = d2a + d2m;
+    if (own) stg_stream(O.p2, M.col + (Off)k * M.sm, two * pc - d.pc + f * lap + damp);
+}
+
+// Step t+1 loads of row k (2D): pp = pcur at row k+1 (every row below the
+// last), the edge lanes' x neighbour, pold, fact and the slab values.
+template <typename T, typename Off, int SA, bool GEN>
+__device__ __forceinline__ NodeLoads<T> two2_load_node(const March<T, Off, 2>& M, int ny, int k,
+                                                       int mside) {
+    NodeLoads<T> q = {};
+    constexpr Off E = (Off)sizeof(T);
+    const Off c = M.col + (Off)k * M.sm;
+    const int lane = (int)threadIdx.x;
+    if (!GEN || k + 1 <= ny - 1) q.pp = ldg(M.pcur, c + M.sm);
+    if (GEN && (k < 1 || k > ny - 2)) return q;
+    if (lane == 0 || lane == kTileX - 1) q.xm = ldg(M.pcur, lane == 0 ? c - E : c + E);
+    q.po = ldg_stream(M.pold, c);
+    q.f = ldg(M.fact, c);
+    if (SA == 2) {
+        q.psa0 = ldg(M.sa.p.psi_rd, M.sa.o_psi + (Off)k * M.sa.st_psi);
+        q.psa1 = ldg(M.sa.p.psi_rd, M.sa.o_psi1 + (Off)k * M.sa.st_psi);
+        q.xia = ldg((const T*)M.sa.p.xi, M.sa.o_xi + (Off)k * M.sa.st_xi);
+    }
+    if (mside >= 0) {
+        const int mz = k - (mside ? M.mshift[1] : M.mshift[0]);
+        const T* pr = mside ? M.pm_side[1].psi_rd : M.pm_side[0].psi_rd;
+        const T* xr = mside ? M.pm_side[1].xi : M.pm_side[0].xi;
+        q.psm0 = ldg(pr, M.col + (Off)mz * M.sm);
+        q.psm1 = ldg(pr, M.col + (Off)(mz - 1) * M.sm);
+        q.xim = ldg(xr, M.col + (Off)(mz - 1) * M.sm);
+    }
+    return q;
+}
+
+template <typename T, typename Off, int SA, bool ZM, bool GEN>
+__device__ __forceinline__ NodeLoads<T> two2_load(const StencilArgs<T>& A,
+                                                  const March<T, Off, 2>& M, int k) {
+    int mz;
+    const int zs = ZM ? classify(k, A.ny, A.halo, A.active[1], &mz) : -1;
+    return two2_load_node<T, Off, SA, GEN>(M, (int)A.ny, k, zs);
+}
+
+template <typename T, typename Off, int SA, bool ZM>
+__device__ __forceinline__ Mid<T> two2_first_at(const StencilArgs<T>& A, const March<T, Off, 2>& M,
+                                                int k, const NodeLoads<T>& q, T pm, T pc, T xm,
+                                                T xp) {
+    if (!ZM) return two2_first<T, Off, SA, -1>(M, 0, q, pm, pc, xm, xp);
+    int mz;
+    const int zs = classify(k, A.ny, A.halo, A.active[1], &mz);
+    if (zs == 0) return two2_first<T, Off, SA, 0>(M, mz, q, pm, pc, xm, xp);
+    if (zs == 1) return two2_first<T, Off, SA, 1>(M, mz, q, pm, pc, xm, xp);
+    return two2_first<T, Off, SA, -1>(M, mz, q, pm, pc, xm, xp);
+}
+
+template <typename T, typename Off, int SA, bool ZM>
+__device__ __forceinline__ void two2_second_at(const StencilArgs<T>& A, const March<T, Off, 2>& M,
+                                               const TwoOut<T, Off>& O, const Mid<T>& d, T pm,
+                                               T pp, T xm, T xp, bool own, int k) {
+    if (!ZM) {
+        two2_second<T, Off, SA, -1>(M, O, d, pm, pp, xm, xp, own, 0, k);
+        return;
+    }
+    int mz;
+    const int zs = classify(k, A.ny, A.halo, A.active[1], &mz);
+    if (zs == 0) two2_second<T, Off, SA, 0>(M, O, d, pm, pp, xm, xp, own, mz, k);
+    else if (zs == 1) two2_second<T, Off, SA, 1>(M, O, d, pm, pp, xm, xp, own, mz, k);
+    else two2_second<T, Off, SA, -1>(M, O, d, pm, pp, xm, xp, own, mz, k);
+}
+
+// One march position k (row) of a 2D wave; GEN as in two_iter.
+template <typename T, typename Off, int PF, int SA, bool ZM, bool GEN, int U>
+__device__ __forceinline__ void two2_iter(const Stencil2Args<T>& B, const March<T, Off, 2>& M,
+                                          const TwoOut<T, Off>& O, const TwoGeom& G,
+                                          TwoCtx<T, PF>& C, int k) {
+    const StencilArgs<T>& A = B.s;
+    const int lane = (int)threadIdx.x;
+    Mid<T> dn = {};
+    const T pp = C.qr[U].pp;
+    if (!GEN || (k >= 1 && k <= G.nz - 2)) {
+        const T pc = C.pz1;
+        const T pcx = G.inx ? pc : T(0);
+        const T sl = lane_shift(pcx, true), sr = lane_shift(pcx, false);
+        const T edge = C.qr[U].xm;
+        const T xm = lane == 0 ? edge : sl;
+        const T xp = lane == kTileX - 1 ? edge : sr;
+        dn = two2_first_at<T, Off, SA, ZM>(A, M, k, C.qr[U], C.pz0, pc, xm, xp);
+        // inject_sources! of step t+1, in source order
+        if (k >= G.ksrc_lo && k <= G.ksrc_hi) {
+            const int64_t lin = G.col_el + (int64_t)k * G.plane;
+            for (int e = G.s0; e < G.s1; ++e)
+                if (B.src_lin[e] == lin) dn.p1 += B.src_wav[(int64_t)e * B.nt + B.it0];
+        }
+        dn.p1 = G.in1 ? dn.p1 : T(0);
+    }
+    C.pz0 = C.pz1;
+    C.pz1 = pp;
+    const int kr = k + PF;
+    if (!GEN || kr <= G.kend) C.qr[U] = two2_load<T, Off, SA, ZM, GEN>(A, M, kr);
+    __builtin_amdgcn_sched_barrier(0);
+    if (G.own && (!GEN || (k >= G.kbeg && k < G.kend)))
+        stg_stream(O.p1, M.col + (Off)k * M.sm, dn.p1);
+    if (!GEN || k - 1 >= G.kbeg) {
+        const T xm = lane_shift(C.d.p1, true);
+        const T xp = lane_shift(C.d.p1, false);
+        two2_second_at<T, Off, SA, ZM>(A, M, O, C.d, C.p1m, dn.p1, xm, xp, G.own, k - 1);
+    }
+    C.p1m = C.d.p1;
+    C.d = dn;
+}
+
+template <typename T, typename Off, int PF, int SA, bool ZM, bool GEN, bool CHECK, int... Us>
+__device__ __forceinline__ void two2_group(const Stencil2Args<T>& B, const March<T, Off, 2>& M,
+                                           const TwoOut<T, Off>& O, const TwoGeom& G,
+                                           TwoCtx<T, PF>& C, int k,
+                                           std::integer_sequence<int, Us...>) {
+    ((!CHECK || k + Us <= G.kend
+          ? (two2_iter<T, Off, PF, SA, ZM, GEN, Us % PF>(B, M, O, G, C, k + Us), 0)
+          : 0),
+     ...);
+}
+
+// The march of one 2D wave: head, steady state, tail as in two_march.
+template <typename T, typename Off, int PF, int SA, bool ZM>
+__device__ __forceinline__ void two2_march(const Stencil2Args<T>& B, const March<T, Off, 2>& M,
+                                           const TwoOut<T, Off>& O, const TwoGeom& G) {
+    const StencilArgs<T>& A = B.s;
+    TwoCtx<T, PF> C;
+    C.pz0 = G.kbeg >= 2 ? ldg(M.pcur, M.col + (Off)(G.kbeg - 2) * M.sm) : T(0);
+    C.pz1 = ldg(M.pcur, M.col + (Off)(G.kbeg - 1) * M.sm);
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+        const int kk = G.kbeg - 1 + u;
+        C.qr[u] = kk <= G.kend ? two2_load<T, Off, SA, ZM, true>(A, M, kk) : NodeLoads<T>{};
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    C.d = Mid<T>{};
+    C.p1m = T(0);
+    C.slot = 0;
+    constexpr int H = (2 + PF - 1) / PF * PF;
+    int k = G.kbeg - 1;
+    two2_group<T, Off, PF, SA, ZM, true, true>(B, M, O, G, C, k, std::make_integer_sequence<int, H>{});
+    k += H;
+    const int last = (G.kend < G.nz - 2 ? G.kend : G.nz - 2) - PF;
+    for (; k + PF - 1 <= last; k += PF)
+        two2_group<T, Off, PF, SA, ZM, false, false>(B, M, O, G, C, k,
+                                                     std::make_integer_sequence<int, PF>{});
+    for (; k <= G.kend; k += PF)
+        two2_group<T, Off, PF, SA, ZM, true, true>(B, M, O, G, C, k,
+                                                   std::make_integer_sequence<int, PF>{});
+}
+
+template <typename T, typename Off, int BY, int PF>
+__global__ __launch_bounds__(64 * BY) void stencil2d_two(const Stencil2Args<T> B) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* lds = reinterpret_cast<T*>(smem_raw);
+    const StencilArgs<T>& A = B.s;
+    const int lb = logical_block(blockIdx.x, A.grid_blocks);
+    if (lb >= A.logical_blocks) return;  // whole block
+    if (A.active[1][0] || A.active[1][1]) stage_march_coef(A, 1, lds);
+    __syncthreads();
+
+    // logical order: x tile fastest, then chunk group, shot
+    int rest = lb;
+    const int tx = rest % A.tiles_x; rest /= A.tiles_x;
+    const int ty = rest % A.tiles_y;
+    const int shot = rest / A.tiles_y;
+    const int wy = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    const int chunk = ty * BY + wy;
+    if (chunk >= A.nchunks) return;  // no barrier follows
+    const int lane = (int)threadIdx.x;
+    const int64_t nx = A.nx, ny = A.ny, halo = A.halo;
+    const int64_t i = (int64_t)tx * kOwnX + lane;
+    const int64_t ic = i < 1 ? 1 : (i > nx - 2 ? nx - 2 : i);
+    const bool in1 = i == ic;
+    const bool own = in1 && lane >= 1 && lane <= kOwnX;
+    const int kbeg = 1 + chunk * A.march_len;
+    const int kend = (int)min((int64_t)kbeg + A.march_len, ny - 1);
+
+    constexpr Off E = (Off)sizeof(T);
+    const int64_t fo = (int64_t)shot * A.field_stride;
+    March<T, Off, 2> M;
+    M.pcur = A.pcur + fo;
+    M.pold = A.pold + fo;
+    M.pnew = nullptr;
+    M.fact = A.fact;
+    M.qout = nullptr;
+    M.sy = (Off)A.px * E;
+    M.sm = M.sy;
+    M.col = (Off)ic * E;
+    M.i2a = A.inv_d2[0];
+    M.i2b = T(0);
+    M.i2m = A.inv_d2[1];
+    M.ia = A.inv_d[0];
+    M.ib = T(0);
+    M.imm = A.inv_d[1];
+    M.lds = lds;
+    M.h1 = (int)halo + 1;
+    M.store_p = true;
+    const Off hp1 = (Off)halo + 1, hl = (Off)halo;
+    // psi_x: (halo+1) x ny, xi_x: halo x ny
+    col_slab<T, Off>(A, 0, ic, nx, shot, 0, E, 0, E, hp1 * E, hl * E, M.sa);
+    M.sb = M.sa;
+    M.sb.in = false;
+    // psi_y: px x (halo+1), xi_y: px x halo
+    for (int s = 0; s < 2; ++s) M.pm_side[s] = slab_ptrs(A, 1, s, shot);
+    M.mshift[0] = 0;
+    M.mshift[1] = (int)(ny - halo - 2);
+
+    TwoOut<T, Off> O;
+    O.p1 = B.p1 + fo;
+    O.p2 = B.p2 + fo;
+    int mx;
+    const int sx = classify(ic, nx, halo, A.active[0], &mx);
+    O.xi_x = (sx == 1 ? B.xi_out[0][1] : B.xi_out[0][0]) + shot * A.xi_stride[0];
+    O.xi_y = nullptr;
+    O.xi_z[0] = B.xi_out[1][0] + shot * A.xi_stride[1];
+    O.xi_z[1] = B.xi_out[1][1] + shot * A.xi_stride[1];
+
+    // rows holding this shot's sources
+    TwoGeom G;
+    G.ksrc_lo = 1 << 30;
+    G.ksrc_hi = -1;
+    G.plane = A.px;
+    G.s0 = B.src_start[shot];
+    G.s1 = B.src_start[shot + 1];
+    for (int e = G.s0; e < G.s1; ++e) {
+        const int kk = (int)(B.src_lin[e] / G.plane);
+        G.ksrc_lo = kk < G.ksrc_lo ? kk : G.ksrc_lo;
+        G.ksrc_hi = kk > G.ksrc_hi ? kk : G.ksrc_hi;
+    }
+    G.col_el = ic;
+    G.wy = wy;
+    G.kbeg = kbeg;
+    G.kend = kend;
+    G.nz = (int)ny;
+    G.in1 = in1;
+    G.own = own;
+    G.inx = in1;
+    G.iny = true;
+
+    const bool xs = __any(M.sa.in);
+    const bool zm = (A.active[1][0] && kbeg - 1 <= halo) || (A.active[1][1] && kend >= ny - halo - 1);
+    if (xs) {
+        if (zm) two2_march<T, Off, PF, 2, true>(B, M, O, G);
+        else two2_march<T, Off, PF, 2, false>(B, M, O, G);
+    } else {
+        if (zm) two2_march<T, Off, PF, 0, true>(B, M, O, G);
+        else two2_march<T, Off, PF, 0, false>(B, M, O, G);
+    }
+}
+
 }  // namespace awcpml

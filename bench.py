@@ -255,6 +255,7 @@ def bench_uneven3d():
 def bench_fwi2d():
     """FWI gradient of the L2 trace misfit (32 shots, 554^2, FP32) with
     checkpointing every 50 steps, and its error against the FP64 gradient."""
+    import awcpml
     from awcpml import fwi
     from cases import fwi2d, rel_l2
     case = fwi2d()
@@ -271,6 +272,7 @@ def bench_fwi2d():
         obs_plan.close()
         P = _shot_plan(S, case["shots"], dtype)
         P.set_model(fact0, S.inv_d, S.inv_d2)
+        awcpml.autotune(P, launches=4)  # forward launch shape, incl. two-step launches
         fwi.misfit_gradient(P, obs, every=case["every"])  # warm-up
         t0 = time.perf_counter()
         J, g = fwi.misfit_gradient(P, obs, every=case["every"])

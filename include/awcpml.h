@@ -158,22 +158,24 @@ int awcpml_get_tiling(const awcpml_plan *plan, int *block_y, int *march_len,
 int awcpml_set_shot_group(awcpml_plan *plan, int shot_group);
 int awcpml_get_shot_group(const awcpml_plan *plan, int *shot_group);
 
-/* Time blocking (3D plans): with steps = 2, awcpml_advance and
+/* Time blocking: with steps = 2, awcpml_advance and
  * awcpml_time_advance advance two time steps per stencil launch wherever
  * that applies, and one step otherwise (an odd remainder, or a plan with a
  * source on a Dirichlet node since its last reset, whose pressure there
  * must rotate through the three field buffers exactly as the reference's
  * does).  Results are bit-identical to one step per launch.  block_rows is
- * the number of wavefronts per block (4, 8 or 16; the block updates
+ * the number of wavefronts per block: in 3D 4, 8 or 16 (the block updates
  * block_rows-2 rows and recomputes the first step on the two outer ones),
- * march_len the z chunk per block, prefetch the planes of loads each
+ * in 2D 4 or 8 (each wavefront marches its own chunk of rows);
+ * march_len the chunk along the march axis (z in 3D, y in 2D), prefetch
+ * the march positions of loads each
  * wavefront keeps in flight (1..3); 0 keeps the library's choice.  The
  * first call with steps = 2 allocates one more field buffer and one more
  * psi and xi buffer per CPML side.  awcpml_get_time_block returns the
  * steps a launch advances now (1 or 2) and the effective block_rows,
  * march_len and prefetch.  A plan that ran two-step launches is reset before it joins a
  * batch.  No reference counterpart (the reference launches one kernel per
- * time step, acoustic_3D_xPU.jl:392-407). */
+ * time step, acoustic_2D_xPU.jl:327-345, acoustic_3D_xPU.jl:392-407). */
 int awcpml_set_time_block(awcpml_plan *plan, int steps, int block_rows, int march_len,
                           int prefetch);
 int awcpml_get_time_block(const awcpml_plan *plan, int *steps, int *block_rows,

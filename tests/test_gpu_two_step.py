@@ -1,4 +1,5 @@
-"""GPU tests of two time steps per stencil launch (awcpml_set_time_block).
+"""GPU tests of two time steps per stencil launch (awcpml_set_time_block),
+3D (block rows exchanged through LDS) and 2D (independent waves).
 
 A two-step launch evaluates each step with the single-step kernel's
 arithmetic, so every result must equal one step per launch bit for bit,
@@ -14,7 +15,7 @@ import numpy as np
 import pytest
 
 import awcpml
-from cases import oracle_problem, small3d
+from cases import oracle_problem, small2d, small3d
 
 pytestmark = pytest.mark.gpu
 
@@ -171,4 +172,91 @@ def test_time_stencil2():
     assert P.time_stencil2(3) > 0.0
     P.reset()
     P.advance(o.nt)
+    P.close()
+
+
+def _interior_2d(n=(57, 43), halo=6, freetop=True, nt=61):
+    """small2d without its source on the top row (a Dirichlet node)."""
+    case = small2d(n=n, halo=halo, freetop=freetop, nt=nt)
+    case["srcpos"] = case["srcpos"][:1]
+    case["t0s"] = case["t0s"][:1]
+    return case
+
+
+CASES_2D = {
+    "freetop": dict(n=(57, 43), halo=6, freetop=True, nt=61),
+    "open": dict(n=(57, 43), halo=6, freetop=False, nt=58),
+    "xtiles": dict(n=(150, 47), halo=5, freetop=True, nt=45),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES_2D))
+@pytest.mark.parametrize("rows,march", [(4, 0), (8, 5), (4, 13)])
+def test_two_step_2d_fp64_bitwise(name, rows, march):
+    print("[gpu] two-step 2D fp64 %s rows %d march %d" % (name, rows, march), flush=True)
+    o = _oracle(_interior_2d(**CASES_2D[name]))
+    o.run()
+    P = _plan(o)
+    _run(P, o.nt, 1)
+    one = _state(P, 1)
+    _run(P, o.nt, 2, rows, march)
+    assert P.time_block()[0] == 2
+    two = _state(P, 1)
+    _same(two, one)
+    assert np.array_equal(two[0][0], o.traces())
+    assert np.array_equal(two[0][2], o.pend())
+    P.close()
+
+
+def test_two_step_2d_fp32_shots_graphs():
+    print("[gpu] two-step 2D fp32, 3 shots, graphs", flush=True)
+    o = _oracle(_interior_2d(n=(150, 61), halo=6, freetop=False, nt=73))
+    ps, _ = o.positions()
+    srcs = []
+    for d in (0, 7, -9):
+        q = ps.copy()
+        q[:, 0] += d
+        srcs.append(q)
+    P = _plan(o, dtype="f32", nshots=3, srcs=srcs)
+    _run(P, o.nt, 1)
+    one = _state(P, 3)
+    for pf in (1, 2, 3):
+        P.set_time_block(2, 4, 9, pf)
+        P.reset()
+        P.advance(6)  # the split leaves an odd remainder: a single step at the end
+        P.advance(o.nt - 6)
+        _same(_state(P, 3), one)
+    P.set_graph(True)
+    for rep in range(2):
+        _run(P, o.nt, 2, 8, 0)
+        _same(_state(P, 3), one)
+    P.set_graph(False)
+    P.close()
+
+
+def test_two_step_2d_falls_back_for_dirichlet_source():
+    print("[gpu] two-step 2D fallback: source on the top row", flush=True)
+    o = _oracle(small2d(nt=40))  # its second source lies on the top row
+    o.run()
+    P = _plan(o)
+    _run(P, o.nt, 2, 4, 0)
+    assert P.time_block()[0] == 1
+    assert np.array_equal(P.traces(), o.traces())
+    assert np.array_equal(P.field(), o.pend())
+    P.close()
+
+
+def test_two_step_fwi_gradient_unchanged():
+    print("[gpu] FWI gradient with two-step forward launches", flush=True)
+    from awcpml import fwi
+    o = _oracle(_interior_2d(n=(57, 43), halo=6, freetop=True, nt=60))
+    o.run()
+    obs = o.traces() * 0.5
+    P = _plan(o)
+    P.set_time_block(1)
+    J1, g1 = fwi.misfit_gradient(P, [obs], every=7)
+    P.set_time_block(2, 4, 9)
+    J2, g2 = fwi.misfit_gradient(P, [obs], every=7)
+    assert J1 == J2
+    assert np.array_equal(g1, g2)
     P.close()
