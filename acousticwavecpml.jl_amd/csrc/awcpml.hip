@@ -161,6 +161,7 @@ struct awcpml_plan {
     void* xi_spare[3][2] = {};
     BufLayout home = {};             // the layout a reset restores
     bool src_on_boundary = false;    // some source lies on a Dirichlet node
+    int src_per_shot_max = 0;
     bool boundary_dirty = false;     // such a source was injected since the reset
 };
 
@@ -420,6 +421,9 @@ int sync_io(awcpml_plan* P) {
     P->n_src_total = (int64_t)slin.size();
     P->n_rec_total = (int64_t)rlin.size();
     P->src_on_boundary = false;
+    P->src_per_shot_max = 0;
+    for (int s = 0; s < P->nshots; ++s)
+        P->src_per_shot_max = std::max(P->src_per_shot_max, sstart[s + 1] - sstart[s]);
     for (int64_t lin : slin) {
         const int64_t i = lin % P->px, j = (lin / P->px) % P->n[1], k = lin / (P->px * P->n[1]);
         bool b = i == 0 || i == P->n[0] - 1 || j == 0 || j == P->n[1] - 1;
@@ -453,9 +457,11 @@ int do_step(awcpml_plan* P, int64_t it0) {
 }
 
 // Two-step launches apply: requested, every field buffer zero on the
-// Dirichlet nodes, 32-bit byte offsets, no d(pnew)/d(fact) output.
+// Dirichlet nodes, at most kMaxSrc2 sources per shot (staged in LDS),
+// 32-bit byte offsets, no d(pnew)/d(fact) output.
 bool two_ok(const awcpml_plan* P) {
     return P->time_block == 2 && P->p_spare && !P->q_cur && !P->src_on_boundary &&
+           P->src_per_shot_max <= awcpml::kMaxSrc2 &&
            !P->boundary_dirty && (uint64_t)P->field_elems * P->esize < (1ull << 32);
 }
 

@@ -37,6 +37,7 @@
 namespace awcpml {
 
 constexpr int kOwnX = kTileX - 2;  // owned x nodes per wave
+constexpr int kMaxSrc2 = 32;       // sources per shot a two-step launch takes
 
 template <typename T>
 struct Stencil2Args {
@@ -206,9 +207,8 @@ __device__ __forceinline__ void two_second(const March<T, Off, 3>& M, const TwoO
 // two planes before, carried in registers); the x neighbours are lane
 // shifts of pc, so only the two edge lanes load one (q.xm: pcur at x-1 on
 // lane 0, at x+1 on lane 63); the y neighbours come from the other waves'
-// columns through LDS, so only the block's outer rows load one (q.ym: pcur
-// at y-1 on row 0, at y+1 on row W-1; one load with a selected offset, as
-// two exclusive loads would again be merged through a selected address).  pp is loaded for every position
+// columns through LDS, so only the block's outer rows need one (q.ym: pcur
+// at y-1 on row 0, at y+1 on row W-1).  pp is loaded for every position
 // below the last plane (the next position needs it as pc); the other loads
 // only for computed planes (1 <= k <= nz-2).  The slot is returned by
 // value: branches that fill different fields of a slot held in memory get
@@ -223,8 +223,11 @@ __device__ __forceinline__ NodeLoads<T> two_load_node(const March<T, Off, 3>& M,
     const int lane = (int)threadIdx.x;
     if (!GEN || k + 1 <= nz - 1) q.pp = ldg(M.pcur, c + M.sm);
     if (GEN && (k < 1 || k > nz - 2)) return q;
-    if (lane == 0 || lane == kTileX - 1) q.xm = ldg(M.pcur, lane == 0 ? c - E : c + E);
-    if (wy == 0 || wy == W - 1) q.ym = ldg(M.pcur, wy == 0 ? c - M.sy : c + M.sy);
+    // issued by every lane and wave (the others reload their own node, a
+    // cache hit), so that each march position issues the same loads and the
+    // waits the compiler places for the ring slots stay exact
+    q.xm = ldg(M.pcur, lane == 0 ? c - E : (lane == kTileX - 1 ? c + E : c));
+    q.ym = ldg(M.pcur, wy == 0 ? c - M.sy : (wy == W - 1 ? c + M.sy : c));
     q.po = ldg_stream(M.pold, c);
     q.f = ldg(M.fact, c);
     if (SA == 2) {
@@ -298,9 +301,33 @@ struct TwoCtx {
     int slot;
 };
 
+// The shot's sources for step t+1, staged in LDS at the start of a launch
+// (their element offsets and wavelet values), so that the march reads them
+// with LDS loads.
+template <typename T>
+struct TwoSrc {
+    int64_t lin[kMaxSrc2];
+    T w[kMaxSrc2];
+};
+
+// Copy the shot's first kMaxSrc2 sources into S (the host launches two-step
+// kernels only for shots with at most that many); returns their count.
+template <typename T>
+__device__ __forceinline__ int two_stage_sources(const Stencil2Args<T>& B, int shot, TwoSrc<T>& S) {
+    const int s0 = B.src_start[shot];
+    int ns = B.src_start[shot + 1] - s0;
+    ns = ns < kMaxSrc2 ? ns : kMaxSrc2;
+    const int tid = (int)(threadIdx.y * blockDim.x + threadIdx.x);
+    if (tid < ns) {
+        S.lin[tid] = B.src_lin[s0 + tid];
+        S.w[tid] = B.src_wav[(int64_t)(s0 + tid) * B.nt + B.it0];
+    }
+    return ns;
+}
+
 // Uniform per-launch values of a wave's march.
 struct TwoGeom {
-    int wy, kbeg, kend, nz, ksrc_lo, ksrc_hi, s0, s1;
+    int wy, kbeg, kend, nz, ksrc_lo, ksrc_hi, ns;
     int64_t col_el, plane;
     bool in1, own;
     bool inx;     // the lane's x is its clamped x (pcur there is the lane's own column)
@@ -315,7 +342,7 @@ template <typename T, typename Off, int W, int PF, int SA, bool SB, bool ZM, boo
 __device__ __forceinline__ void two_iter(const Stencil2Args<T>& B, const March<T, Off, 3>& M,
                                          const TwoOut<T, Off>& O, T (*ring)[W][kTileX],
                                          T (*pring)[W][kTileX], const TwoGeom& G,
-                                         TwoCtx<T, PF>& C, int k) {
+                                         const TwoSrc<T>& S, TwoCtx<T, PF>& C, int k) {
     const StencilArgs<T>& A = B.s;
     const int lane = (int)threadIdx.x;
     Mid<T> dn = {};
@@ -335,8 +362,8 @@ __device__ __forceinline__ void two_iter(const Stencil2Args<T>& B, const March<T
         // inject_sources! of step t+1, in source order
         if (k >= G.ksrc_lo && k <= G.ksrc_hi) {
             const int64_t lin = G.col_el + (int64_t)k * G.plane;
-            for (int e = G.s0; e < G.s1; ++e)
-                if (B.src_lin[e] == lin) dn.p1 += B.src_wav[(int64_t)e * B.nt + B.it0];
+            for (int e = 0; e < G.ns; ++e)
+                if (S.lin[e] == lin) dn.p1 += S.w[e];
         }
         dn.p1 = G.in1 ? dn.p1 : T(0);
     }
@@ -375,10 +402,11 @@ template <typename T, typename Off, int W, int PF, int SA, bool SB, bool ZM, boo
 __device__ __forceinline__ void two_group(const Stencil2Args<T>& B, const March<T, Off, 3>& M,
                                           const TwoOut<T, Off>& O, T (*ring)[W][kTileX],
                                           T (*pring)[W][kTileX], const TwoGeom& G,
-                                          TwoCtx<T, PF>& C, int k,
+                                          const TwoSrc<T>& S, TwoCtx<T, PF>& C, int k,
                                           std::integer_sequence<int, Us...>) {
     ((!CHECK || k + Us <= G.kend
-          ? (two_iter<T, Off, W, PF, SA, SB, ZM, GEN, Us % PF>(B, M, O, ring, pring, G, C, k + Us),
+          ? (two_iter<T, Off, W, PF, SA, SB, ZM, GEN, Us % PF>(B, M, O, ring, pring, G, S, C,
+                                                               k + Us),
              0)
           : 0),
      ...);
@@ -393,7 +421,8 @@ __device__ __forceinline__ void two_group(const Stencil2Args<T>& B, const March<
 template <typename T, typename Off, int W, int PF, int SA, bool SB, bool ZM>
 __device__ __forceinline__ void two_march(const Stencil2Args<T>& B, const March<T, Off, 3>& M,
                                           const TwoOut<T, Off>& O, T (*ring)[W][kTileX],
-                                          T (*pring)[W][kTileX], const TwoGeom& G) {
+                                          T (*pring)[W][kTileX], const TwoGeom& G,
+                                          const TwoSrc<T>& S) {
     const StencilArgs<T>& A = B.s;
     TwoCtx<T, PF> C;
     C.pz0 = G.kbeg >= 2 ? ldg(M.pcur, M.col + (Off)(G.kbeg - 2) * M.sm) : T(0);
@@ -411,15 +440,15 @@ __device__ __forceinline__ void two_march(const Stencil2Args<T>& B, const March<
     __syncthreads();
     constexpr int H = (2 + PF - 1) / PF * PF;
     int k = G.kbeg - 1;
-    two_group<T, Off, W, PF, SA, SB, ZM, true, true>(B, M, O, ring, pring, G, C, k,
+    two_group<T, Off, W, PF, SA, SB, ZM, true, true>(B, M, O, ring, pring, G, S, C, k,
                                                      std::make_integer_sequence<int, H>{});
     k += H;
     const int last = (G.kend < G.nz - 2 ? G.kend : G.nz - 2) - PF;  // last steady position
     for (; k + PF - 1 <= last; k += PF)
-        two_group<T, Off, W, PF, SA, SB, ZM, false, false>(B, M, O, ring, pring, G, C, k,
+        two_group<T, Off, W, PF, SA, SB, ZM, false, false>(B, M, O, ring, pring, G, S, C, k,
                                                            std::make_integer_sequence<int, PF>{});
     for (; k <= G.kend; k += PF)
-        two_group<T, Off, W, PF, SA, SB, ZM, true, true>(B, M, O, ring, pring, G, C, k,
+        two_group<T, Off, W, PF, SA, SB, ZM, true, true>(B, M, O, ring, pring, G, S, C, k,
                                                          std::make_integer_sequence<int, PF>{});
 }
 
@@ -427,13 +456,12 @@ template <typename T, typename Off, int W, int PF>
 __global__ __launch_bounds__(64 * W) void stencil3d_two(const Stencil2Args<T> B) {
     __shared__ T ring[3][W][kTileX];    // p^{t+1} planes
     __shared__ T pring[3][W][kTileX];   // pcur planes
+    __shared__ TwoSrc<T> S;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* lds = reinterpret_cast<T*>(smem_raw);
     const StencilArgs<T>& A = B.s;
     const int lb = logical_block(blockIdx.x, A.grid_blocks);
     if (lb >= A.logical_blocks) return;  // whole block
-    if (A.active[2][0] || A.active[2][1]) stage_march_coef(A, 2, lds);
-    __syncthreads();
 
     // logical order: x tile fastest, then y tile, chunk, shot
     int rest = lb;
@@ -441,6 +469,9 @@ __global__ __launch_bounds__(64 * W) void stencil3d_two(const Stencil2Args<T> B)
     const int ty = rest % A.tiles_y; rest /= A.tiles_y;
     const int chunk = rest % A.nchunks;
     const int shot = rest / A.nchunks;
+    const int ns = two_stage_sources(B, shot, S);
+    if (A.active[2][0] || A.active[2][1]) stage_march_coef(A, 2, lds);
+    __syncthreads();
     const int wy = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     const int lane = (int)threadIdx.x;
     const int64_t nx = A.nx, ny = A.ny, nz = A.nz, halo = A.halo;
@@ -501,10 +532,9 @@ __global__ __launch_bounds__(64 * W) void stencil3d_two(const Stencil2Args<T> B)
     G.ksrc_lo = 1 << 30;
     G.ksrc_hi = -1;
     G.plane = A.px * ny;
-    G.s0 = B.src_start[shot];
-    G.s1 = B.src_start[shot + 1];
-    for (int e = G.s0; e < G.s1; ++e) {
-        const int kk = (int)(B.src_lin[e] / G.plane);
+    G.ns = ns;
+    for (int e = 0; e < ns; ++e) {
+        const int kk = (int)(S.lin[e] / G.plane);
         G.ksrc_lo = kk < G.ksrc_lo ? kk : G.ksrc_lo;
         G.ksrc_hi = kk > G.ksrc_hi ? kk : G.ksrc_hi;
     }
@@ -523,7 +553,7 @@ __global__ __launch_bounds__(64 * W) void stencil3d_two(const Stencil2Args<T> B)
     // any z slab layer among the planes this chunk reads (block-uniform)
     const bool zm = (A.active[2][0] && kbeg - 1 <= halo) || (A.active[2][1] && kend >= nz - halo - 1);
 #define AWCPML_TWO(SA_, SB_, ZM_)                                                             \
-    two_march<T, Off, W, PF, SA_, SB_, ZM_>(B, M, O, ring, pring, G)
+    two_march<T, Off, W, PF, SA_, SB_, ZM_>(B, M, O, ring, pring, G, S)
     if (zm) {
         if (xs) {
             if (ys) AWCPML_TWO(2, true, true);
@@ -629,7 +659,7 @@ __device__ __forceinline__ NodeLoads<T> two2_load_node(const March<T, Off, 2>& M
     const int lane = (int)threadIdx.x;
     if (!GEN || k + 1 <= ny - 1) q.pp = ldg(M.pcur, c + M.sm);
     if (GEN && (k < 1 || k > ny - 2)) return q;
-    if (lane == 0 || lane == kTileX - 1) q.xm = ldg(M.pcur, lane == 0 ? c - E : c + E);
+    q.xm = ldg(M.pcur, lane == 0 ? c - E : (lane == kTileX - 1 ? c + E : c));
     q.po = ldg_stream(M.pold, c);
     q.f = ldg(M.fact, c);
     if (SA == 2) {
@@ -687,7 +717,7 @@ __device__ __forceinline__ void two2_second_at(const StencilArgs<T>& A, const Ma
 template <typename T, typename Off, int PF, int SA, bool ZM, bool GEN, int U>
 __device__ __forceinline__ void two2_iter(const Stencil2Args<T>& B, const March<T, Off, 2>& M,
                                           const TwoOut<T, Off>& O, const TwoGeom& G,
-                                          TwoCtx<T, PF>& C, int k) {
+                                          const TwoSrc<T>& S, TwoCtx<T, PF>& C, int k) {
     const StencilArgs<T>& A = B.s;
     const int lane = (int)threadIdx.x;
     Mid<T> dn = {};
@@ -703,8 +733,8 @@ __device__ __forceinline__ void two2_iter(const Stencil2Args<T>& B, const March<
         // inject_sources! of step t+1, in source order
         if (k >= G.ksrc_lo && k <= G.ksrc_hi) {
             const int64_t lin = G.col_el + (int64_t)k * G.plane;
-            for (int e = G.s0; e < G.s1; ++e)
-                if (B.src_lin[e] == lin) dn.p1 += B.src_wav[(int64_t)e * B.nt + B.it0];
+            for (int e = 0; e < G.ns; ++e)
+                if (S.lin[e] == lin) dn.p1 += S.w[e];
         }
         dn.p1 = G.in1 ? dn.p1 : T(0);
     }
@@ -727,10 +757,10 @@ __device__ __forceinline__ void two2_iter(const Stencil2Args<T>& B, const March<
 template <typename T, typename Off, int PF, int SA, bool ZM, bool GEN, bool CHECK, int... Us>
 __device__ __forceinline__ void two2_group(const Stencil2Args<T>& B, const March<T, Off, 2>& M,
                                            const TwoOut<T, Off>& O, const TwoGeom& G,
-                                           TwoCtx<T, PF>& C, int k,
+                                           const TwoSrc<T>& S, TwoCtx<T, PF>& C, int k,
                                            std::integer_sequence<int, Us...>) {
     ((!CHECK || k + Us <= G.kend
-          ? (two2_iter<T, Off, PF, SA, ZM, GEN, Us % PF>(B, M, O, G, C, k + Us), 0)
+          ? (two2_iter<T, Off, PF, SA, ZM, GEN, Us % PF>(B, M, O, G, S, C, k + Us), 0)
           : 0),
      ...);
 }
@@ -738,7 +768,8 @@ __device__ __forceinline__ void two2_group(const Stencil2Args<T>& B, const March
 // The march of one 2D wave: head, steady state, tail as in two_march.
 template <typename T, typename Off, int PF, int SA, bool ZM>
 __device__ __forceinline__ void two2_march(const Stencil2Args<T>& B, const March<T, Off, 2>& M,
-                                           const TwoOut<T, Off>& O, const TwoGeom& G) {
+                                           const TwoOut<T, Off>& O, const TwoGeom& G,
+                                           const TwoSrc<T>& S) {
     const StencilArgs<T>& A = B.s;
     TwoCtx<T, PF> C;
     C.pz0 = G.kbeg >= 2 ? ldg(M.pcur, M.col + (Off)(G.kbeg - 2) * M.sm) : T(0);
@@ -754,32 +785,34 @@ __device__ __forceinline__ void two2_march(const Stencil2Args<T>& B, const March
     C.slot = 0;
     constexpr int H = (2 + PF - 1) / PF * PF;
     int k = G.kbeg - 1;
-    two2_group<T, Off, PF, SA, ZM, true, true>(B, M, O, G, C, k, std::make_integer_sequence<int, H>{});
+    two2_group<T, Off, PF, SA, ZM, true, true>(B, M, O, G, S, C, k, std::make_integer_sequence<int, H>{});
     k += H;
     const int last = (G.kend < G.nz - 2 ? G.kend : G.nz - 2) - PF;
     for (; k + PF - 1 <= last; k += PF)
-        two2_group<T, Off, PF, SA, ZM, false, false>(B, M, O, G, C, k,
+        two2_group<T, Off, PF, SA, ZM, false, false>(B, M, O, G, S, C, k,
                                                      std::make_integer_sequence<int, PF>{});
     for (; k <= G.kend; k += PF)
-        two2_group<T, Off, PF, SA, ZM, true, true>(B, M, O, G, C, k,
+        two2_group<T, Off, PF, SA, ZM, true, true>(B, M, O, G, S, C, k,
                                                    std::make_integer_sequence<int, PF>{});
 }
 
 template <typename T, typename Off, int BY, int PF>
 __global__ __launch_bounds__(64 * BY) void stencil2d_two(const Stencil2Args<T> B) {
+    __shared__ TwoSrc<T> S;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* lds = reinterpret_cast<T*>(smem_raw);
     const StencilArgs<T>& A = B.s;
     const int lb = logical_block(blockIdx.x, A.grid_blocks);
     if (lb >= A.logical_blocks) return;  // whole block
-    if (A.active[1][0] || A.active[1][1]) stage_march_coef(A, 1, lds);
-    __syncthreads();
 
     // logical order: x tile fastest, then chunk group, shot
     int rest = lb;
     const int tx = rest % A.tiles_x; rest /= A.tiles_x;
     const int ty = rest % A.tiles_y;
     const int shot = rest / A.tiles_y;
+    const int ns = two_stage_sources(B, shot, S);
+    if (A.active[1][0] || A.active[1][1]) stage_march_coef(A, 1, lds);
+    __syncthreads();
     const int wy = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     const int chunk = ty * BY + wy;
     if (chunk >= A.nchunks) return;  // no barrier follows
@@ -837,10 +870,9 @@ __global__ __launch_bounds__(64 * BY) void stencil2d_two(const Stencil2Args<T> B
     G.ksrc_lo = 1 << 30;
     G.ksrc_hi = -1;
     G.plane = A.px;
-    G.s0 = B.src_start[shot];
-    G.s1 = B.src_start[shot + 1];
-    for (int e = G.s0; e < G.s1; ++e) {
-        const int kk = (int)(B.src_lin[e] / G.plane);
+    G.ns = ns;
+    for (int e = 0; e < ns; ++e) {
+        const int kk = (int)(S.lin[e] / G.plane);
         G.ksrc_lo = kk < G.ksrc_lo ? kk : G.ksrc_lo;
         G.ksrc_hi = kk > G.ksrc_hi ? kk : G.ksrc_hi;
     }
@@ -857,11 +889,11 @@ __global__ __launch_bounds__(64 * BY) void stencil2d_two(const Stencil2Args<T> B
     const bool xs = __any(M.sa.in);
     const bool zm = (A.active[1][0] && kbeg - 1 <= halo) || (A.active[1][1] && kend >= ny - halo - 1);
     if (xs) {
-        if (zm) two2_march<T, Off, PF, 2, true>(B, M, O, G);
-        else two2_march<T, Off, PF, 2, false>(B, M, O, G);
+        if (zm) two2_march<T, Off, PF, 2, true>(B, M, O, G, S);
+        else two2_march<T, Off, PF, 2, false>(B, M, O, G, S);
     } else {
-        if (zm) two2_march<T, Off, PF, 0, true>(B, M, O, G);
-        else two2_march<T, Off, PF, 0, false>(B, M, O, G);
+        if (zm) two2_march<T, Off, PF, 0, true>(B, M, O, G, S);
+        else two2_march<T, Off, PF, 0, false>(B, M, O, G, S);
     }
 }
 
