@@ -206,6 +206,15 @@ void set_layout(awcpml_plan* P, const BufLayout& L) {
     P->par = L.par;
 }
 
+// The plan's buffers play their home roles (no two-step launch swapped
+// them since the last reset); batches precompute arguments for that layout.
+bool at_home(const awcpml_plan* P) {
+    BufLayout now = get_layout(P);
+    now.rot = P->home.rot;
+    now.par = P->home.par;
+    return now == P->home;
+}
+
 int dalloc(void** ptr, size_t bytes) {
     if (bytes == 0) bytes = 256;
     HIP_TRY(hipMalloc(ptr, bytes));
@@ -1347,10 +1356,7 @@ int awcpml_batch_create(awcpml_plan* const* plans, int count, awcpml_batch** out
             return fail(AWCPML_ERR_ARG, "plans of a batch must share ndim and dtype (plan %d differs)", q);
         int st = ready(plans[q]);
         if (st) return st;
-        BufLayout now = get_layout(plans[q]);
-        now.rot = plans[q]->home.rot;
-        now.par = plans[q]->home.par;
-        if (!(now == plans[q]->home))
+        if (!at_home(plans[q]))
             return fail(AWCPML_ERR_STATE, "plan %d ran two-step launches: reset it before batching", q);
         if (plans[q]->block_y != plans[0]->block_y || plans[q]->prefetch != plans[0]->prefetch)
             return fail(AWCPML_ERR_ARG, "plans of a batch must share block_y and prefetch (plan %d differs)", q);
@@ -1411,7 +1417,7 @@ int awcpml_batch_advance(awcpml_batch* B, int64_t nsteps) {
     if (!B) return fail(AWCPML_ERR_ARG, "batch is NULL");
     if (nsteps < 0) return fail(AWCPML_ERR_ARG, "nsteps < 0");
     for (auto* P : B->plans)
-        if (P->it != std::min(B->it, P->nt) || P->io_dirty)
+        if (P->it != std::min(B->it, P->nt) || P->io_dirty || !at_home(P))
             return fail(AWCPML_ERR_STATE, "a plan of the batch was stepped or changed on its own; "
                                           "rebuild the batch or call awcpml_batch_reset");
     if (B->it + nsteps > B->nt_max)
