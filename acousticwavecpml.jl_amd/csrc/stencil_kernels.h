@@ -40,9 +40,9 @@
 //
 // Blocks are mapped XCD-aware: the hardware hands block b to XCD b % 8, and
 // logical_block() renumbers so that each XCD walks one contiguous range of
-// tiles (shot fastest, then x tile, y tile, march chunk).  Neighbouring
-// tiles, which share halo lines, and the shots of one tile, which share
-// `fact`, then run on the same XCD and meet in its L2.
+// logical tiles (x tile fastest, then y tile, march chunk, shot group).
+// Neighbouring tiles, which share halo lines, then run on the same XCD at
+// about the same time and meet in its L2.
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -235,7 +235,7 @@ int awcpml_batch_time_stencil(awcpml_batch *batch, int nlaunches, double *ms_tot
  * receivers change.  This is the forward half of a reverse-mode rule: the
  * reference has no gradient yet (README.md:297 lists the adjoint and
  * checkpointing as work in progress); the Julia side would expose the pair
- * as a ChainRulesCore rrule of the solve. */
+ * (this call and awcpml_adjoint) as a ChainRulesCore rrule of the solve. */
 int awcpml_checkpoint_run(awcpml_plan *plan, int64_t every);
 
 /* Pullback of the traces: given trace_bar = dJ/dtraces (for every shot in
