@@ -71,6 +71,24 @@ __device__ __forceinline__ double lane_shift(double v, bool below) {
     return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
+// A copy of v in a register of its own (early-clobber output).  A value
+// read from a ring slot that must outlive the slot's refill (pp, which
+// becomes the next plane's pc, and fact, which step t+2 reads) is copied
+// out before the refill.  The refill can then load straight into the
+// slot's registers.  Otherwise the register allocator places the loads
+// elsewhere and copies them into the slot at the loop latch, and those
+// copies wait for every load in flight.
+__device__ __forceinline__ float own_reg(float v) {
+    float r;
+    asm("v_mov_b32 %0, %1" : "=&v"(r) : "v"(v));
+    return r;
+}
+__device__ __forceinline__ double own_reg(double v) {
+    double r;
+    asm("v_mov_b64 %0, %1" : "=&v"(r) : "v"(v));
+    return r;
+}
+
 // Step t+1 results of one node that step t+2 at the same node reads.
 template <typename T>
 struct Mid {
@@ -143,7 +161,7 @@ __device__ __forceinline__ Mid<T> two_first(const March<T, Off, 3>& M, int mz,
     const T lap = d2a + d2b + d2m;  // (d2x + d2y) + d2z
     r.p1 = two * pc - q.po + f * lap + damp;
     r.pc = pc;
-    r.f = f;
+    r.f = own_reg(f);
     return r;
 }
 
@@ -346,7 +364,7 @@ __device__ __forceinline__ void two_iter(const Stencil2Args<T>& B, const March<T
     const StencilArgs<T>& A = B.s;
     const int lane = (int)threadIdx.x;
     Mid<T> dn = {};
-    const T pp = C.qr[U].pp;
+    const T pp = own_reg(C.qr[U].pp);
     if (!GEN || (k >= 1 && k <= G.nz - 2)) {
         // x neighbours: lane shifts of pc; a lane outside the grid's x range
         // (clamped) stands for the Dirichlet node there, 0
@@ -444,6 +462,11 @@ __device__ __forceinline__ void two_march(const Stencil2Args<T>& B, const March<
                                                      std::make_integer_sequence<int, H>{});
     k += H;
     const int last = (G.kend < G.nz - 2 ? G.kend : G.nz - 2) - PF;  // last steady position
+    // drain the head's memory operations once: the steady loop is then
+    // entered with nothing in flight, and the counter waits the compiler
+    // derives at its head follow the loop's own issue order (merged with the
+    // head's different order they fall back to waiting for everything)
+    __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
     for (; k + PF - 1 <= last; k += PF)
         two_group<T, Off, W, PF, SA, SB, ZM, false, false>(B, M, O, ring, pring, G, S, C, k,
                                                            std::make_integer_sequence<int, PF>{});
@@ -610,7 +633,7 @@ __device__ __forceinline__ Mid<T> two2_first(const March<T, Off, 2>& M, int mz,
     const T lap = d2a + d2m;  // d2x + d2y
     r.p1 = two * pc - q.po + f * lap + damp;
     r.pc = pc;
-    r.f = f;
+    r.f = own_reg(f);
     return r;
 }
 
@@ -721,7 +744,7 @@ __device__ __forceinline__ void two2_iter(const Stencil2Args<T>& B, const March<
     const StencilArgs<T>& A = B.s;
     const int lane = (int)threadIdx.x;
     Mid<T> dn = {};
-    const T pp = C.qr[U].pp;
+    const T pp = own_reg(C.qr[U].pp);
     if (!GEN || (k >= 1 && k <= G.nz - 2)) {
         const T pc = C.pz1;
         const T pcx = G.inx ? pc : T(0);
