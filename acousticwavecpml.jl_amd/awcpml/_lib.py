@@ -62,6 +62,9 @@ SIGNATURES = {
     "awcpml_batch_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
     "awcpml_batch_time_advance": (ctypes.c_int, [_vp, ctypes.c_int64, _dp]),
     "awcpml_batch_time_stencil": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "awcpml_batch_set_time_block": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "awcpml_batch_get_time_block": (ctypes.c_int, [_vp] + [ctypes.POINTER(ctypes.c_int)] * 3),
+    "awcpml_batch_time_stencil2": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "awcpml_checkpoint_run": (ctypes.c_int, [_vp, ctypes.c_int64]),
     "awcpml_adjoint": (ctypes.c_int, [_vp, _dp, ctypes.c_int64, _dp]),
     "awcpml_last_error": (ctypes.c_char_p, []),

@@ -54,6 +54,17 @@ def two_step_candidates(plan):
     return out
 
 
+def two_step_batch_candidates(plans):
+    """(block_rows, march_len, prefetch) of two-step batch launches worth
+    timing; march_len is applied to every plan."""
+    if plans[0].ndim == 2:
+        return [(w, ml, pf) for w in (4, 8) for ml in (32, 64) for pf in (1, 2)]
+    if plans[0].dtype == 0:
+        return [(8, ml, pf) for ml in (24, 48) for pf in (1, 2)] + [(4, 24, 2)]
+    return ([(16, ml, pf) for ml in (24, 48, 96, 144) for pf in (1, 2)]
+            + [(8, ml, 2) for ml in (24, 48)])
+
+
 def _apply(plan, best):
     plan.set_tiling(best["block_y"], best["march_len"], best["prefetch"])
     plan.set_shot_group(best.get("shot_group", 1))
@@ -132,13 +143,19 @@ def autotune(plan, cands=None, launches=8, cache=DEFAULT_CACHE, use_cache=True):
     return dict(best, cached=False)
 
 
-def autotune_batch(plans, cands=None, launches=3):
+def autotune_batch(plans, cands=None, launches=3, two_step=True):
     """Pick the launch shape (block_y, march_len, prefetch) shared by a batch
     of plans: each candidate is applied to every plan, a Batch is built and
-    its stencil launch timed.  Returns the best shape and leaves it applied
-    to the plans (build the Batch afterwards)."""
+    its stencil launch timed.  Two-step batch launches (two time steps of
+    every plan per launch) are then timed per time step and kept when they
+    are faster.  Returns the best shape and leaves it applied to the plans:
+    a Batch built from them afterwards uses it."""
+    from . import _lib
     from .solver import Batch
     cands = cands or [(by, ml, pf) for by in (2, 4) for ml in (32, 64, 128) for pf in (2, 3, 5)]
+    for P in plans:
+        P.set_time_block(1)
+        P.reset()  # the wavefields are overwritten by the timing anyway
     results = []
     for by, ml, pf in cands:
         for P in plans:
@@ -153,6 +170,35 @@ def autotune_batch(plans, cands=None, launches=3):
     ms, by, ml, pf = min(results)
     for P in plans:
         P.set_tiling(by, ml, pf)
+    best = {"block_y": by, "march_len": ml, "prefetch": pf, "ms_per_launch": ms,
+            "ms_per_step": ms, "time_block": None, "tried": len(results)}
+    two = []
+    for w, ml2, pf2 in (two_step_batch_candidates(plans) if two_step else []):
+        try:
+            for P in plans:
+                P.set_time_block(2, w, ml2, pf2)
+            B = Batch(plans)
+            try:
+                if B.time_block()[0] != 2:
+                    break  # two-step launches do not apply to some plan
+                B.reset()
+                B.time_stencil2(1)
+                n2 = max(1, launches // 2)
+                two.append((B.time_stencil2(n2) / n2 / 2.0, w, ml2, pf2))
+            finally:
+                B.close()
+        except _lib.AwcpmlError:
+            break
+    best["tried"] += len(two)
+    if two and min(two)[0] < best["ms_per_step"]:
+        ms2, w, ml2, pf2 = min(two)
+        best["time_block"] = {"block_rows": w, "march_len": ml2, "prefetch": pf2}
+        best["ms_per_step"] = ms2
+        for P in plans:
+            P.set_time_block(2, w, ml2, pf2)
+    else:
+        for P in plans:
+            P.set_time_block(1)
+    for P in plans:
         P.reset()
-    return {"block_y": by, "march_len": ml, "prefetch": pf, "ms_per_launch": ms,
-            "tried": len(results)}
+    return best

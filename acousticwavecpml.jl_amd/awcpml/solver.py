@@ -265,6 +265,25 @@ class Batch:
         _lib.check(_lib.load().awcpml_batch_time_stencil(self._h, int(nlaunches), ctypes.byref(ms)))
         return ms.value
 
+    def set_time_block(self, steps, block_rows=0, prefetch=0):
+        """Two time steps of every plan per stencil launch (steps=2) while
+        every plan has two steps left; each plan keeps its own march_len.
+        Results are bit-identical to single steps."""
+        _lib.check(_lib.load().awcpml_batch_set_time_block(self._h, int(steps), int(block_rows),
+                                                           int(prefetch)))
+
+    def time_block(self):
+        """(steps per launch now, block_rows, prefetch)."""
+        v = [ctypes.c_int() for _ in range(3)]
+        _lib.check(_lib.load().awcpml_batch_get_time_block(self._h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def time_stencil2(self, nlaunches):
+        """Milliseconds of nlaunches two-step launches alone (reset afterwards)."""
+        ms = ctypes.c_double()
+        _lib.check(_lib.load().awcpml_batch_time_stencil2(self._h, int(nlaunches), ctypes.byref(ms)))
+        return ms.value
+
 
 def _effective_bytes(n, halo, esize):
     """A_eff of the reference's do_bench report (acoustic_2D_xPU.jl:288-294,

@@ -6,7 +6,7 @@ The library is linked from several objects compiled in parallel:
   awcpml.hip          plan/batch management, the C ABI, source/receiver and
                       adjoint kernels
   launch_kernels.hip  the stencil kernels, once per (precision, dimension,
-                      plan or batch) via -DAWCPML_REAL/-DAWCPML_NDIM/-DAWCPML_BATCH
+                      plan, batch or two-step batch) via -DAWCPML_REAL/-DAWCPML_NDIM/-DAWCPML_BATCH
 
 -ffp-contract=off keeps every multiply and add separately rounded so the
 FP64 kernels reproduce the reference's Float64 arithmetic operation for
@@ -39,8 +39,8 @@ def _units():
     units = [("awcpml", os.path.join(CSRC, "awcpml.hip"), [])]
     for real in ("float", "double"):
         for ndim in (2, 3):
-            for batch in (0, 1):
-                name = "launch_%s_%dd_%s" % (real, ndim, "batch" if batch else "plan")
+            for batch in (0, 1, 2):
+                name = "launch_%s_%dd_%s" % (real, ndim, ("plan", "batch", "batch2")[batch])
                 units.append((name, os.path.join(CSRC, "launch_kernels.hip"),
                               ["-DAWCPML_REAL=" + real, "-DAWCPML_NDIM=%d" % ndim,
                                "-DAWCPML_BATCH=%d" % batch]))

@@ -83,7 +83,8 @@ struct BufLayout {
 struct GraphEntry {
     int64_t it0, n;
     hipGraphExec_t exec;
-    BufLayout before, after;  // plans only
+    BufLayout before, after;            // plans only
+    int roles_before, roles_after;      // batches only
 };
 
 void drop_graphs(std::vector<GraphEntry>& g) {
@@ -465,31 +466,36 @@ int do_step(awcpml_plan* P, int64_t it0) {
     return AWCPML_OK;
 }
 
-// Two-step launches apply: requested, every field buffer zero on the
-// Dirichlet nodes, at most kMaxSrc2 sources per shot (staged in LDS),
-// 32-bit byte offsets, no d(pnew)/d(fact) output.
-bool two_ok(const awcpml_plan* P) {
-    return P->time_block == 2 && P->p_spare && !P->q_cur && !P->src_on_boundary &&
+// Two-step launches can run: spare buffers allocated, every field buffer
+// zero on the Dirichlet nodes, at most kMaxSrc2 sources per shot (staged
+// in LDS), 32-bit byte offsets, no d(pnew)/d(fact) output.  two_ok: they
+// also were requested for the plan.
+bool two_applies(const awcpml_plan* P) {
+    return P->p_spare && !P->q_cur && !P->src_on_boundary &&
            P->src_per_shot_max <= awcpml::kMaxSrc2 &&
            !P->boundary_dirty && (uint64_t)P->field_elems * P->esize < (1ull << 32);
 }
 
+bool two_ok(const awcpml_plan* P) { return P->time_block == 2 && two_applies(P); }
+
+int two_rows_default(int ndim, int dtype) { return ndim == 2 ? 4 : (dtype == AWCPML_F64 ? 8 : 16); }
+
+// The plan's chunk length of two-step launches with W waves per block.
+int two_march(const awcpml_plan* P, int W) {
+    if (P->tb_march > 0) return P->tb_march;
+    if (P->ndim == 2) return (int)std::max<int64_t>(8, std::min<int64_t>(64, (P->n[1] - 2) / 16));
+    // enough blocks to fill 256 CUs several times over
+    const int64_t tiles = ((P->n[0] - 2 + awcpml::kOwnX - 1) / awcpml::kOwnX) *
+                          ((P->n[1] - 2 + W - 3) / (W - 2)) * P->nshots;
+    const int64_t interior = P->n[2] - 2;
+    const int64_t want = std::max<int64_t>(1, (2048 + tiles - 1) / tiles);
+    return (int)std::max<int64_t>(16, std::min<int64_t>((interior + want - 1) / want, 128));
+}
+
 void two_tiling(const awcpml_plan* P, int* rows, int* march, int* prefetch) {
-    int W = P->tb_rows;
-    if (W <= 0) W = P->ndim == 2 ? 4 : (P->dtype == AWCPML_F64 ? 8 : 16);
-    int ml = P->tb_march;
-    if (ml <= 0 && P->ndim == 2) {
-        ml = (int)std::max<int64_t>(8, std::min<int64_t>(64, (P->n[1] - 2) / 16));
-    } else if (ml <= 0) {
-        // enough blocks to fill 256 CUs several times over
-        const int64_t tiles = ((P->n[0] - 2 + awcpml::kOwnX - 1) / awcpml::kOwnX) *
-                              ((P->n[1] - 2 + W - 3) / (W - 2)) * P->nshots;
-        const int64_t interior = P->n[2] - 2;
-        const int64_t want = std::max<int64_t>(1, (2048 + tiles - 1) / tiles);
-        ml = (int)std::max<int64_t>(16, std::min<int64_t>((interior + want - 1) / want, 128));
-    }
+    const int W = P->tb_rows > 0 ? P->tb_rows : two_rows_default(P->ndim, P->dtype);
     *rows = W;
-    *march = ml;
+    *march = two_march(P, W);
     *prefetch = P->tb_prefetch > 0 ? P->tb_prefetch : 2;
 }
 
@@ -634,16 +640,6 @@ int capture_seq(hipStream_t stream, Fn fn, hipGraphExec_t* out) {
     return capture_steps(stream, 1, [&](int64_t) { return fn(); }, out);
 }
 
-GraphEntry* find_graph(std::vector<GraphEntry>& g, int64_t it0, int64_t n) {
-    for (auto& e : g)
-        if (e.it0 == it0 && e.n == n) return &e;
-    if (g.size() >= 4) {
-        (void)hipGraphExecDestroy(g.front().exec);
-        g.erase(g.begin());
-    }
-    return nullptr;
-}
-
 // The plan's graph for n steps from step it0 and layout L, captured on
 // first use.  The host state (layout, boundary flag) after the capture is
 // the state after the sequence; it is recorded and the plan restored to L.
@@ -724,57 +720,229 @@ int check_positions(const awcpml_plan* P, const int64_t* pos, int count, const c
 
 // ---------------------------------------------------------------------
 // Batches: several plans (different grids, same ndim and precision)
-// advanced by one stencil launch and one source/receiver launch per step.
+// advanced by one stencil launch and one source/receiver launch per step,
+// or per two steps with two-step launches.
 // ---------------------------------------------------------------------
+namespace {
+
+// Buffer roles of a batch's plans relative to their home layouts: field
+// slot b (0..2: the rotation, 3: the spare) holds home buffer p[b], psi
+// slot c (0..1: the parities, 2: the spare) home psi[c], xi slot c (1: the
+// spare) home xi[c].  A two-step launch permutes them the same way in every
+// plan of the batch.
+struct Roles {
+    int p[4], psi[3], xi[2];
+    bool operator==(const Roles& o) const {
+        for (int b = 0; b < 4; ++b)
+            if (p[b] != o.p[b]) return false;
+        for (int c = 0; c < 3; ++c)
+            if (psi[c] != o.psi[c]) return false;
+        return xi[0] == o.xi[0] && xi[1] == o.xi[1];
+    }
+};
+
+Roles home_roles() { return Roles{{0, 1, 2, 3}, {0, 1, 2}, {0, 1}}; }
+
+// The roles after a two-step launch of steps it0, it0+1 (two_swap).
+Roles roles_after_two(Roles R, int64_t it0) {
+    std::swap(R.p[it0 % 3], R.p[3]);
+    std::swap(R.psi[it0 % 2], R.psi[2]);
+    std::swap(R.xi[0], R.xi[1]);
+    return R;
+}
+
+// The layout of plan P after `it` steps with buffer roles R.
+BufLayout roles_layout(const awcpml_plan* P, const Roles& R, int64_t it) {
+    BufLayout L = P->home;
+    for (int b = 0; b < 4; ++b) L.p[b] = P->home.p[R.p[b]];
+    for (int a = 0; a < 3; ++a)
+        for (int s = 0; s < 2; ++s) {
+            for (int c = 0; c < 3; ++c) L.psi[a][s][c] = P->home.psi[a][s][R.psi[c]];
+            for (int c = 0; c < 2; ++c) L.xi[a][s][c] = P->home.xi[a][s][R.xi[c]];
+        }
+    L.rot = (int)(it % 3);
+    L.par = (int)(it % 2);
+    return L;
+}
+
+// Device argument tables of every plan for one set of roles.
+struct RoleTables {
+    Roles r;
+    void* d_args = nullptr;   // 6 (step mod 6) x nplans single-step arguments
+    void* d_args2 = nullptr;  // 6 (step mod 6) x nplans two-step arguments, or null
+    void* d_io = nullptr;     // 3 (rotation) x 2 (inject) x nplans io arguments
+};
+
+constexpr int kMaxRoles = 288;  // 4! * 3! * 2!: every permutation of the roles
+
+}  // namespace
+
 struct awcpml_batch {
     std::vector<awcpml_plan*> plans;
     int ndim = 0, dtype = 0, block_y = 4, prefetch = 2, halo_max = 0;
     bool use32 = true, has_io = false;
     hipStream_t stream = nullptr;
-    void* d_args = nullptr;  // 6 rotation variants x nplans stencil arguments
-    void* d_io = nullptr;    // 3 rotation variants x nplans io arguments
-    std::vector<int> block_start, shot_start;
-    int grid_blocks = 0;
-    int64_t it = 0, nt_max = 0;
+    std::vector<RoleTables> roles;   // built on first use, kept until destroy
+    int cur = 0;                     // roles[cur]: the plans' roles now
+    int time_block = 1, tb_rows = 0, tb_prefetch = 0;
+    std::vector<int> block_start, block_start2, shot_start;
+    int grid_blocks = 0, grid_blocks2 = 0;
+    int64_t it = 0, nt_max = 0, nt_min = 0;
     bool use_graph = false;
     std::vector<GraphEntry> graphs;
 };
 
 namespace {
 
+void batch_two_shape(const awcpml_batch* B, int* rows, int* prefetch) {
+    *rows = B->tb_rows > 0 ? B->tb_rows : two_rows_default(B->ndim, B->dtype);
+    *prefetch = B->tb_prefetch > 0 ? B->tb_prefetch : 2;
+}
+
+// Two-step batch launches apply (roles tables without two-step arguments
+// were built while some plan had no spare buffers).
+bool batch_two_ok(const awcpml_batch* B) {
+    if (B->time_block != 2 || !B->roles[B->cur].d_args2) return false;
+    for (auto* P : B->plans)
+        if (!two_applies(P)) return false;
+    return true;
+}
+
 template <typename T>
-int batch_build_t(awcpml_batch* B) {
+int batch_tables_t(awcpml_batch* B, const Roles& R, RoleTables* out) {
     const int np = (int)B->plans.size();
+    bool two = true;
+    for (auto* P : B->plans) two = two && P->p_spare;
+    int W, pf;
+    batch_two_shape(B, &W, &pf);
     std::vector<StencilArgs<T>> args((size_t)6 * np);
-    std::vector<awcpml::IoArgs<T>> io((size_t)3 * np);
-    for (int v = 0; v < 6; ++v)
-        for (int q = 0; q < np; ++q) args[(size_t)v * np + q] = make_args<T>(B->plans[q], v % 3, v % 2);
-    for (int r = 0; r < 3; ++r)
-        for (int q = 0; q < np; ++q) io[(size_t)r * np + q] = make_io<T>(B->plans[q], r);
+    std::vector<awcpml::Stencil2Args<T>> args2(two ? (size_t)6 * np : 0);
+    std::vector<awcpml::IoArgs<T>> io((size_t)6 * np);
+    for (int q = 0; q < np; ++q) {
+        awcpml_plan* P = B->plans[q];
+        const BufLayout keep = get_layout(P);
+        set_layout(P, roles_layout(P, R, 0));
+        for (int v = 0; v < 6; ++v) args[(size_t)v * np + q] = make_args<T>(P, v % 3, v % 2);
+        if (two) {
+            const int ml = two_march(P, W);
+            for (int v = 0; v < 6; ++v) {
+                P->rot = v % 3;
+                P->par = v % 2;
+                args2[(size_t)v * np + q] = make_args2<T>(P, 0, W, ml);
+            }
+        }
+        for (int r = 0; r < 3; ++r)
+            for (int inj = 0; inj < 2; ++inj) {
+                awcpml::IoArgs<T> A = make_io<T>(P, r);
+                A.inject = inj;
+                io[(size_t)(r * 2 + inj) * np + q] = A;
+            }
+        set_layout(P, keep);
+    }
+    if (two) {
+        B->block_start2.assign(1, 0);
+        for (int q = 0; q < np; ++q) B->block_start2.push_back(B->block_start2.back() + args2[q].s.logical_blocks);
+        B->grid_blocks2 = (int)round_up(B->block_start2.back(), awcpml::kNumXcd);
+    }
+    RoleTables t;
+    t.r = R;
+    int st;
+    if ((st = dalloc(&t.d_args, args.size() * sizeof(StencilArgs<T>)))) return st;
+    if (two && (st = dalloc(&t.d_args2, args2.size() * sizeof(awcpml::Stencil2Args<T>)))) {
+        dfree(t.d_args);
+        return st;
+    }
+    if ((st = dalloc(&t.d_io, io.size() * sizeof(awcpml::IoArgs<T>)))) {
+        dfree(t.d_args);
+        dfree(t.d_args2);
+        return st;
+    }
+    *out = t;
+    HIP_TRY(hipMemcpyAsync(t.d_args, args.data(), args.size() * sizeof(StencilArgs<T>),
+                           hipMemcpyHostToDevice, B->stream));
+    if (two)
+        HIP_TRY(hipMemcpyAsync(t.d_args2, args2.data(), args2.size() * sizeof(awcpml::Stencil2Args<T>),
+                               hipMemcpyHostToDevice, B->stream));
+    HIP_TRY(hipMemcpyAsync(t.d_io, io.data(), io.size() * sizeof(awcpml::IoArgs<T>),
+                           hipMemcpyHostToDevice, B->stream));
+    HIP_TRY(hipStreamSynchronize(B->stream));
+    return AWCPML_OK;
+}
+
+void free_tables(RoleTables& t) {
+    dfree(t.d_args);
+    dfree(t.d_args2);
+    dfree(t.d_io);
+}
+
+// Index of the tables for roles R, built on first use (allocates: call
+// outside stream capture).
+int batch_roles(awcpml_batch* B, const Roles& R, int* idx) {
+    for (size_t q = 0; q < B->roles.size(); ++q)
+        if (B->roles[q].r == R) {
+            *idx = (int)q;
+            return AWCPML_OK;
+        }
+    if ((int)B->roles.size() >= kMaxRoles) return fail(AWCPML_ERR_STATE, "batch roles table full");
+    RoleTables t;
+    int st = B->dtype == AWCPML_F64 ? batch_tables_t<double>(B, R, &t) : batch_tables_t<float>(B, R, &t);
+    if (st) {
+        free_tables(t);
+        return st;
+    }
+    B->roles.push_back(t);
+    *idx = (int)B->roles.size() - 1;
+    return AWCPML_OK;
+}
+
+// Drop every table and graph and rebuild the tables of the current roles.
+int batch_rebuild(awcpml_batch* B) {
+    HIP_TRY(hipStreamSynchronize(B->stream));
+    drop_graphs(B->graphs);
+    const Roles R = B->roles.empty() ? home_roles() : B->roles[B->cur].r;
+    for (auto& t : B->roles) free_tables(t);
+    B->roles.clear();
+    B->cur = 0;
+    return batch_roles(B, R, &B->cur);
+}
+
+int batch_build(awcpml_batch* B) {
+    const int np = (int)B->plans.size();
     B->block_start.assign(1, 0);
     B->shot_start.assign(1, 0);
     for (int q = 0; q < np; ++q) {
-        B->block_start.push_back(B->block_start.back() + args[q].logical_blocks);
-        B->shot_start.push_back(B->shot_start.back() + B->plans[q]->nshots);
+        const awcpml_plan* P = B->plans[q];
+        int lb;
+        if (B->dtype == AWCPML_F64) lb = make_args<double>(P, 0, 0).logical_blocks;
+        else lb = make_args<float>(P, 0, 0).logical_blocks;
+        B->block_start.push_back(B->block_start.back() + lb);
+        B->shot_start.push_back(B->shot_start.back() + P->nshots);
     }
     B->grid_blocks = (int)round_up(B->block_start.back(), awcpml::kNumXcd);
-    int st;
-    if ((st = dalloc(&B->d_args, args.size() * sizeof(StencilArgs<T>)))) return st;
-    if ((st = dalloc(&B->d_io, io.size() * sizeof(awcpml::IoArgs<T>)))) return st;
-    HIP_TRY(hipMemcpyAsync(B->d_args, args.data(), args.size() * sizeof(StencilArgs<T>),
-                           hipMemcpyHostToDevice, B->stream));
-    HIP_TRY(hipMemcpyAsync(B->d_io, io.data(), io.size() * sizeof(awcpml::IoArgs<T>),
-                           hipMemcpyHostToDevice, B->stream));
-    HIP_TRY(hipStreamSynchronize(B->stream));
+    return batch_rebuild(B);
+}
+
+template <typename T>
+int batch_io_t(awcpml_batch* B, const RoleTables& R, int rot, int inject, int64_t it0) {
+    const int np = (int)B->plans.size();
+    awcpml::IoBatchArgs<T> I;
+    memset(&I, 0, sizeof I);
+    I.table = (const awcpml::IoArgs<T>*)R.d_io + (size_t)(rot * 2 + inject) * np;
+    I.it0 = it0;
+    I.nplans = np;
+    for (int q = 0; q <= np; ++q) I.shot_start[q] = B->shot_start[q];
+    awcpml::sources_receivers_batch<T><<<dim3(B->shot_start[np]), dim3(256), 0, B->stream>>>(I);
+    HIP_TRY(hipGetLastError());
     return AWCPML_OK;
 }
 
 template <typename T>
 int batch_step_t(awcpml_batch* B, int64_t it0, bool with_io) {
     const int np = (int)B->plans.size();
+    const RoleTables& R = B->roles[B->cur];
     awcpml::BatchArgs<T> A;
     memset(&A, 0, sizeof A);
-    A.table = (const StencilArgs<T>*)B->d_args + (size_t)(it0 % 6) * np;
+    A.table = (const StencilArgs<T>*)R.d_args + (size_t)(it0 % 6) * np;
     A.it0 = it0;
     A.nplans = np;
     A.grid_blocks = B->grid_blocks;
@@ -788,16 +956,7 @@ int batch_step_t(awcpml_batch* B, int64_t it0, bool with_io) {
     if (B->ndim == 3) awcpml::launch_batch_stencil<T, 3>(A, L, B->prefetch, !B->use32);
     else awcpml::launch_batch_stencil<T, 2>(A, L, B->prefetch, !B->use32);
     HIP_TRY(hipGetLastError());
-    if (with_io && B->has_io) {
-        awcpml::IoBatchArgs<T> I;
-        memset(&I, 0, sizeof I);
-        I.table = (const awcpml::IoArgs<T>*)B->d_io + (size_t)(it0 % 3) * np;
-        I.it0 = it0;
-        I.nplans = np;
-        for (int q = 0; q <= np; ++q) I.shot_start[q] = B->shot_start[q];
-        awcpml::sources_receivers_batch<T><<<dim3(B->shot_start[np]), dim3(256), 0, B->stream>>>(I);
-        HIP_TRY(hipGetLastError());
-    }
+    if (with_io && B->has_io) return batch_io_t<T>(B, R, (int)(it0 % 3), 1, it0);
     return AWCPML_OK;
 }
 
@@ -806,36 +965,141 @@ int batch_step(awcpml_batch* B, int64_t it0, bool with_io = true) {
                                   : batch_step_t<float>(B, it0, with_io);
 }
 
-// After the batch ran to step B->it, every plan is at min(it, its nt).
+template <typename T>
+int batch_launch2_t(awcpml_batch* B, const RoleTables& R, int64_t it0) {
+    const int np = (int)B->plans.size();
+    if (!R.d_args2) return fail(AWCPML_ERR_STATE, "batch roles without two-step arguments");
+    int W, pf;
+    batch_two_shape(B, &W, &pf);
+    awcpml::Batch2Args<T> A;
+    memset(&A, 0, sizeof A);
+    A.table = (const awcpml::Stencil2Args<T>*)R.d_args2 + (size_t)(it0 % 6) * np;
+    A.it0 = it0;
+    A.nplans = np;
+    A.grid_blocks = B->grid_blocks2;
+    for (int q = 0; q <= np; ++q) A.block_start[q] = B->block_start2[q];
+    awcpml::LaunchShape L;
+    L.grid = dim3(B->grid_blocks2);
+    L.block = dim3(awcpml::kTileX, W);
+    L.lds = (size_t)8 * (B->halo_max + 1) * sizeof(T);
+    L.stream = B->stream;
+    if (B->ndim == 3) awcpml::launch_batch_stencil2<T, 3>(A, L, W, pf);
+    else awcpml::launch_batch_stencil2<T, 2>(A, L, W, pf);
+    HIP_TRY(hipGetLastError());
+    return AWCPML_OK;
+}
+
+// Steps it0 and it0+1 of every plan in one launch, then as do_step2: the
+// receivers of step it0 read p^{t+1}, and step it0+1's sources and
+// receivers are handled on p^{t+2} under the new roles (whose tables must
+// exist when this is captured).
+template <typename T>
+int batch_step2_t(awcpml_batch* B, int64_t it0) {
+    const int before = B->cur;
+    int st = batch_launch2_t<T>(B, B->roles[before], it0);
+    if (st) return st;
+    if (B->has_io && (st = batch_io_t<T>(B, B->roles[before], (int)(it0 % 3), 0, it0))) return st;
+    if ((st = batch_roles(B, roles_after_two(B->roles[before].r, it0), &B->cur))) return st;
+    if (B->has_io) return batch_io_t<T>(B, B->roles[B->cur], (int)((it0 + 1) % 3), 1, it0 + 1);
+    return AWCPML_OK;
+}
+
+// Pairs of steps run in two-step launches while every plan has two steps
+// left, so that the roles change only while the whole batch runs.
+bool batch_pair(const awcpml_batch* B, bool two, int64_t it0, int64_t left) {
+    return two && left >= 2 && it0 + 2 <= B->nt_min;
+}
+
+// Enqueue steps it0 .. it0+n-1, advancing B->cur as they are enqueued.
+int batch_enqueue(awcpml_batch* B, int64_t it0, int64_t n) {
+    const bool two = batch_two_ok(B);
+    for (int64_t q = 0; q < n;) {
+        int st;
+        if (batch_pair(B, two, it0 + q, n - q)) {
+            st = B->dtype == AWCPML_F64 ? batch_step2_t<double>(B, it0 + q) : batch_step2_t<float>(B, it0 + q);
+            q += 2;
+        } else {
+            st = batch_step(B, it0 + q);
+            q += 1;
+        }
+        if (st) return st;
+    }
+    return AWCPML_OK;
+}
+
+// Build the tables of every roles entry that steps it0 .. it0+n-1 reach, so
+// that capturing them allocates nothing.
+int batch_prepare(awcpml_batch* B, int64_t it0, int64_t n) {
+    if (!batch_two_ok(B)) return AWCPML_OK;
+    Roles R = B->roles[B->cur].r;
+    for (int64_t q = 0; q < n;) {
+        if (batch_pair(B, true, it0 + q, n - q)) {
+            R = roles_after_two(R, it0 + q);
+            int idx, st = batch_roles(B, R, &idx);
+            if (st) return st;
+            q += 2;
+        } else {
+            q += 1;
+        }
+    }
+    return AWCPML_OK;
+}
+
+// After the batch ran to step B->it, every plan is at min(it, its nt), in
+// the batch's roles (which change only while every plan runs).
 void batch_sync_plans(awcpml_batch* B) {
     for (auto* P : B->plans) {
         P->it = std::min(B->it, P->nt);
-        P->rot = (int)(P->it % 3);
-        P->par = (int)(P->it % 2);
+        set_layout(P, roles_layout(P, B->roles[B->cur].r, P->it));
     }
+}
+
+// The batch's graph for n steps from its current step and roles, captured
+// on first use.
+int batch_graph(awcpml_batch* B, int64_t n, GraphEntry** out) {
+    const int64_t it0 = B->it;
+    const int before = B->cur;
+    for (auto& e : B->graphs)
+        if (e.it0 == it0 && e.n == n && e.roles_before == before) {
+            *out = &e;
+            return AWCPML_OK;
+        }
+    int st = batch_prepare(B, it0, n);
+    if (st) return st;
+    if (B->graphs.size() >= 4) {
+        (void)hipGraphExecDestroy(B->graphs.front().exec);
+        B->graphs.erase(B->graphs.begin());
+    }
+    hipGraphExec_t exec;
+    st = capture_seq(B->stream, [&]() { return batch_enqueue(B, it0, n); }, &exec);
+    const int after = B->cur;
+    B->cur = before;
+    if (st) return st;
+    GraphEntry e = {};
+    e.it0 = it0;
+    e.n = n;
+    e.exec = exec;
+    e.roles_before = before;
+    e.roles_after = after;
+    B->graphs.push_back(e);
+    *out = &B->graphs.back();
+    return AWCPML_OK;
 }
 
 int batch_run_steps(awcpml_batch* B, int64_t n) {
     if (n <= 0) return AWCPML_OK;
     if (B->use_graph) {
-        GraphEntry* g = find_graph(B->graphs, B->it, n);
-        if (!g) {
-            const int64_t it0 = B->it;
-            hipGraphExec_t exec;
-            int st = capture_steps(B->stream, n, [&](int64_t q) { return batch_step(B, it0 + q); }, &exec);
-            if (st) return st;
-            B->graphs.push_back({it0, n, exec});
-            g = &B->graphs.back();
-        }
+        GraphEntry* g = nullptr;
+        int st = batch_graph(B, n, &g);
+        if (st) return st;
         HIP_TRY(hipGraphLaunch(g->exec, B->stream));
         B->it += n;
+        B->cur = g->roles_after;
         return AWCPML_OK;
     }
-    for (int64_t q = 0; q < n; ++q) {
-        int st = batch_step(B, B->it);
-        if (st) return st;
-        ++B->it;
-    }
+    int st = batch_enqueue(B, B->it, n);
+    if (st) return st;
+    B->it += n;
     return AWCPML_OK;
 }
 
@@ -1368,18 +1632,30 @@ int awcpml_batch_create(awcpml_plan* const* plans, int count, awcpml_batch** out
     B->dtype = plans[0]->dtype;
     B->block_y = plans[0]->block_y;
     B->prefetch = plans[0]->prefetch;
+    B->nt_min = plans[0]->nt;
     for (auto* P : B->plans) {
         B->halo_max = std::max(B->halo_max, P->halo);
         B->nt_max = std::max(B->nt_max, P->nt);
+        B->nt_min = std::min(B->nt_min, P->nt);
         B->use32 = B->use32 && (uint64_t)P->field_elems * P->esize < (1ull << 32);
         B->has_io = B->has_io || P->has_io;
+    }
+    // two-step launches when every plan asks for them with one launch shape
+    bool two = true;
+    for (auto* P : B->plans)
+        two = two && P->time_block == 2 && P->tb_rows == plans[0]->tb_rows &&
+              P->tb_prefetch == plans[0]->tb_prefetch;
+    if (two) {
+        B->time_block = 2;
+        B->tb_rows = plans[0]->tb_rows;
+        B->tb_prefetch = plans[0]->tb_prefetch;
     }
     hipError_t e = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete B;
         return fail(AWCPML_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
     }
-    int st = B->dtype == AWCPML_F64 ? batch_build_t<double>(B) : batch_build_t<float>(B);
+    int st = batch_build(B);
     if (st) {
         std::string keep = g_err;
         awcpml_batch_destroy(B);
@@ -1394,8 +1670,7 @@ int awcpml_batch_destroy(awcpml_batch* B) {
     if (!B) return AWCPML_OK;
     if (B->stream) (void)hipStreamSynchronize(B->stream);
     drop_graphs(B->graphs);
-    dfree(B->d_args);
-    dfree(B->d_io);
+    for (auto& t : B->roles) free_tables(t);
     if (B->stream) (void)hipStreamDestroy(B->stream);
     delete B;
     return AWCPML_OK;
@@ -1409,7 +1684,7 @@ int awcpml_batch_reset(awcpml_batch* B) {
         if (st) return st;
     }
     B->it = 0;
-    return AWCPML_OK;
+    return batch_roles(B, home_roles(), &B->cur);
 }
 
 int awcpml_batch_advance(awcpml_batch* B, int64_t nsteps) {
@@ -1417,7 +1692,8 @@ int awcpml_batch_advance(awcpml_batch* B, int64_t nsteps) {
     if (!B) return fail(AWCPML_ERR_ARG, "batch is NULL");
     if (nsteps < 0) return fail(AWCPML_ERR_ARG, "nsteps < 0");
     for (auto* P : B->plans)
-        if (P->it != std::min(B->it, P->nt) || P->io_dirty || !at_home(P))
+        if (P->it != std::min(B->it, P->nt) || P->io_dirty ||
+            !(get_layout(P) == roles_layout(P, B->roles[B->cur].r, P->it)))
             return fail(AWCPML_ERR_STATE, "a plan of the batch was stepped or changed on its own; "
                                           "rebuild the batch or call awcpml_batch_reset");
     if (B->it + nsteps > B->nt_max)
@@ -1445,12 +1721,11 @@ int awcpml_batch_time_advance(awcpml_batch* B, int64_t nsteps, double* ms_total)
     if (!B || !ms_total || nsteps < 0) return fail(AWCPML_ERR_ARG, "bad argument");
     if (B->it + nsteps > B->nt_max) return fail(AWCPML_ERR_STATE, "advance past the longest run");
     int st;
-    if (B->use_graph && nsteps > 0 && !find_graph(B->graphs, B->it, nsteps)) {
-        const int64_t it0 = B->it;
-        hipGraphExec_t exec;
-        st = capture_steps(B->stream, nsteps, [&](int64_t q) { return batch_step(B, it0 + q); }, &exec);
-        if (st) return st;
-        B->graphs.push_back({it0, nsteps, exec});
+    if (nsteps > 0 && B->use_graph) {  // capture outside the timed region
+        GraphEntry* g = nullptr;
+        if ((st = batch_graph(B, nsteps, &g))) return st;
+    } else if ((st = batch_prepare(B, B->it, nsteps))) {
+        return st;
     }
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
@@ -1477,6 +1752,56 @@ int awcpml_batch_time_stencil(awcpml_batch* B, int nlaunches, double* ms_total) 
     HIP_TRY(hipEventRecord(e0, B->stream));
     int st = AWCPML_OK;
     for (int q = 0; q < nlaunches && !st; ++q) st = batch_step(B, q % 6, false);
+    HIP_TRY(hipEventRecord(e1, B->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_total = ms;
+    return st;
+}
+
+int awcpml_batch_set_time_block(awcpml_batch* B, int steps, int block_rows, int prefetch) {
+    g_err.clear();
+    if (!B) return fail(AWCPML_ERR_ARG, "batch is NULL");
+    if (steps != 1 && steps != 2) return fail(AWCPML_ERR_ARG, "steps must be 1 or 2 (got %d)", steps);
+    if (B->ndim == 3 && block_rows != 0 && block_rows != 4 && block_rows != 8 && block_rows != 16)
+        return fail(AWCPML_ERR_ARG, "block_rows must be 0, 4, 8 or 16 for 3D plans (got %d)", block_rows);
+    if (B->ndim == 2 && block_rows != 0 && block_rows != 4 && block_rows != 8)
+        return fail(AWCPML_ERR_ARG, "block_rows must be 0, 4 or 8 for 2D plans (got %d)", block_rows);
+    if (prefetch < 0 || prefetch > 3) return fail(AWCPML_ERR_ARG, "prefetch must be 0..3 (got %d)", prefetch);
+    HIP_TRY(hipStreamSynchronize(B->stream));
+    // each plan keeps its own chunk length; spares are allocated where missing
+    for (auto* P : B->plans) {
+        int st = awcpml_set_time_block(P, steps, block_rows, P->tb_march, prefetch);
+        if (st) return st;
+    }
+    B->time_block = steps;
+    B->tb_rows = block_rows;
+    B->tb_prefetch = prefetch;
+    return batch_rebuild(B);
+}
+
+int awcpml_batch_get_time_block(const awcpml_batch* B, int* steps, int* block_rows, int* prefetch) {
+    if (!B || !steps || !block_rows || !prefetch) return fail(AWCPML_ERR_ARG, "NULL argument");
+    *steps = batch_two_ok(B) ? 2 : 1;
+    batch_two_shape(B, block_rows, prefetch);
+    return AWCPML_OK;
+}
+
+int awcpml_batch_time_stencil2(awcpml_batch* B, int nlaunches, double* ms_total) {
+    g_err.clear();
+    if (!B || !ms_total || nlaunches < 1) return fail(AWCPML_ERR_ARG, "bad argument");
+    if (!batch_two_ok(B)) return fail(AWCPML_ERR_STATE, "two-step launches do not apply to this batch");
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, B->stream));
+    int st = AWCPML_OK;
+    const RoleTables& R = B->roles[B->cur];
+    for (int q = 0; q < nlaunches && !st; ++q)
+        st = B->dtype == AWCPML_F64 ? batch_launch2_t<double>(B, R, 0) : batch_launch2_t<float>(B, R, 0);
     HIP_TRY(hipEventRecord(e1, B->stream));
     HIP_TRY(hipEventSynchronize(e1));
     float ms = 0.f;

@@ -1,7 +1,7 @@
 // launch.h -- host-side launchers of the stencil kernels.  Each (precision,
-// dimension, plan/batch) combination is instantiated in its own object file
-// (launch_kernels.hip, compiled once per combination by build.py), so the
-// heavily templated kernels compile in parallel.
+// dimension, plan / batch / two-step batch) combination is instantiated in
+// its own object file (launch_kernels.hip, compiled once per combination by
+// build.py), so the heavily templated kernels compile in parallel.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -30,5 +30,8 @@ void launch_batch_stencil(const BatchArgs<T>& A, const LaunchShape& L, int pipe,
 // per wave (1..3).
 template <typename T, int NDIM>
 void launch_plan_stencil2(const Stencil2Args<T>& B, const LaunchShape& L, int rows, int pf);
+
+template <typename T, int NDIM>
+void launch_batch_stencil2(const Batch2Args<T>& B, const LaunchShape& L, int rows, int pf);
 
 }  // namespace awcpml

@@ -1,11 +1,11 @@
 // launch_kernels.hip -- definitions of the launchers declared in launch.h,
 // instantiated for AWCPML_REAL (float or double), AWCPML_NDIM (2 or 3) and
-// AWCPML_BATCH (0: one plan, 1: a batch of plans), given on the compiler
-// command line (see build.py).
+// AWCPML_BATCH (0: one plan, 1: a batch of plans, 2: a batch of plans, two
+// steps per launch), given on the compiler command line (see build.py).
 #include "launch.h"
 
 #if !defined(AWCPML_REAL) || !defined(AWCPML_NDIM) || !defined(AWCPML_BATCH)
-#error "compile with -DAWCPML_REAL=<float|double> -DAWCPML_NDIM=<2|3> -DAWCPML_BATCH=<0|1>"
+#error "compile with -DAWCPML_REAL=<float|double> -DAWCPML_NDIM=<2|3> -DAWCPML_BATCH=<0|1|2>"
 #endif
 
 namespace awcpml {
@@ -67,7 +67,7 @@ void launch_plan_stencil2(const Stencil2Args<T>& B, const LaunchShape& L, int ro
 }
 template void launch_plan_stencil2<AWCPML_REAL, AWCPML_NDIM>(const Stencil2Args<AWCPML_REAL>&,
                                                             const LaunchShape&, int, int);
-#else
+#elif AWCPML_BATCH == 1
 template <typename T, typename Off, int P, int NDIM>
 static void go_batch(const BatchArgs<T>& A, const LaunchShape& L) {
     stencil_batch<T, Off, P, NDIM><<<L.grid, L.block, L.lds, L.stream>>>(A);
@@ -89,6 +89,41 @@ void launch_batch_stencil(const BatchArgs<T>& A, const LaunchShape& L, int pipe,
 }
 template void launch_batch_stencil<AWCPML_REAL, AWCPML_NDIM>(const BatchArgs<AWCPML_REAL>&,
                                                             const LaunchShape&, int, bool);
+#else
+template <typename T, int W>
+static void go_two_batch(const Batch2Args<T>& B, const LaunchShape& L, int pf) {
+#if AWCPML_NDIM == 3
+    switch (pf) {
+        case 1: stencil3d_two_batch<T, uint32_t, W, 1><<<L.grid, L.block, L.lds, L.stream>>>(B); break;
+        case 3: stencil3d_two_batch<T, uint32_t, W, 3><<<L.grid, L.block, L.lds, L.stream>>>(B); break;
+        default: stencil3d_two_batch<T, uint32_t, W, 2><<<L.grid, L.block, L.lds, L.stream>>>(B); break;
+    }
+#else
+    switch (pf) {
+        case 1: stencil2d_two_batch<T, uint32_t, W, 1><<<L.grid, L.block, L.lds, L.stream>>>(B); break;
+        case 3: stencil2d_two_batch<T, uint32_t, W, 3><<<L.grid, L.block, L.lds, L.stream>>>(B); break;
+        default: stencil2d_two_batch<T, uint32_t, W, 2><<<L.grid, L.block, L.lds, L.stream>>>(B); break;
+    }
+#endif
+}
+
+template <typename T, int NDIM>
+void launch_batch_stencil2(const Batch2Args<T>& B, const LaunchShape& L, int rows, int pf) {
+#if AWCPML_NDIM == 3
+    switch (rows) {
+        case 4: go_two_batch<T, 4>(B, L, pf); break;
+        case 16: go_two_batch<T, 16>(B, L, pf); break;
+        default: go_two_batch<T, 8>(B, L, pf); break;
+    }
+#else
+    switch (rows) {
+        case 8: go_two_batch<T, 8>(B, L, pf); break;
+        default: go_two_batch<T, 4>(B, L, pf); break;
+    }
+#endif
+}
+template void launch_batch_stencil2<AWCPML_REAL, AWCPML_NDIM>(const Batch2Args<AWCPML_REAL>&,
+                                                             const LaunchShape&, int, int);
 #endif
 
 }  // namespace awcpml

@@ -269,31 +269,41 @@ __device__ __forceinline__ NodeLoads<T> two_load_node(const March<T, Off, 3>& M,
     return q;
 }
 
+// Uniform per-launch values of a wave's march.
+struct TwoGeom {
+    int wy, kbeg, kend, nz, ksrc_lo, ksrc_hi, ns;
+    int halo, act[2];  // march axis: CPML layers, active sides (no argument reads in the march)
+    int64_t col_el, plane;
+    bool in1, own;
+    bool inx;     // the lane's x is its clamped x (pcur there is the lane's own column)
+    bool iny;     // the wave's y is its clamped y
+};
+
 // ZM = false: no z slab layer in the chunk.  One code path for both sides
 // (a branch per side would be merged into stores through selected
 // addresses, which keeps the slot in scratch memory).
 template <typename T, typename Off, int W, int SA, bool SB, bool ZM, bool GEN>
-__device__ __forceinline__ NodeLoads<T> two_load(const StencilArgs<T>& A,
-                                                 const March<T, Off, 3>& M, int k, int wy) {
+__device__ __forceinline__ NodeLoads<T> two_load(const TwoGeom& G, const March<T, Off, 3>& M,
+                                                 int k, int wy) {
     int mz;
-    const int zs = ZM ? classify(k, A.nz, A.halo, A.active[2], &mz) : -1;
-    return two_load_node<T, Off, W, SA, SB, GEN>(M, (int)A.nz, k, zs, wy);
+    const int zs = ZM ? classify(k, G.nz, G.halo, G.act, &mz) : -1;
+    return two_load_node<T, Off, W, SA, SB, GEN>(M, G.nz, k, zs, wy);
 }
 
 template <typename T, typename Off, int SA, bool SB, bool ZM>
-__device__ __forceinline__ Mid<T> two_first_at(const StencilArgs<T>& A, const March<T, Off, 3>& M,
+__device__ __forceinline__ Mid<T> two_first_at(const TwoGeom& G, const March<T, Off, 3>& M,
                                                int k, const NodeLoads<T>& q, T pm, T pc, T xm,
                                                T xp, T ym, T yp) {
     if (!ZM) return two_first<T, Off, SA, SB, -1>(M, 0, q, pm, pc, xm, xp, ym, yp);
     int mz;
-    const int zs = classify(k, A.nz, A.halo, A.active[2], &mz);
+    const int zs = classify(k, G.nz, G.halo, G.act, &mz);
     if (zs == 0) return two_first<T, Off, SA, SB, 0>(M, mz, q, pm, pc, xm, xp, ym, yp);
     if (zs == 1) return two_first<T, Off, SA, SB, 1>(M, mz, q, pm, pc, xm, xp, ym, yp);
     return two_first<T, Off, SA, SB, -1>(M, mz, q, pm, pc, xm, xp, ym, yp);
 }
 
 template <typename T, typename Off, int SA, bool SB, bool ZM>
-__device__ __forceinline__ void two_second_at(const StencilArgs<T>& A, const March<T, Off, 3>& M,
+__device__ __forceinline__ void two_second_at(const TwoGeom& G, const March<T, Off, 3>& M,
                                               const TwoOut<T, Off>& O, const Mid<T>& d, T pm,
                                               T pp, T xm, T xp, T ym, T yp, bool own, int k) {
     if (!ZM) {
@@ -301,7 +311,7 @@ __device__ __forceinline__ void two_second_at(const StencilArgs<T>& A, const Mar
         return;
     }
     int mz;
-    const int zs = classify(k, A.nz, A.halo, A.active[2], &mz);
+    const int zs = classify(k, G.nz, G.halo, G.act, &mz);
     if (zs == 0) two_second<T, Off, SA, SB, 0>(M, O, d, pm, pp, xm, xp, ym, yp, own, mz, k);
     else if (zs == 1) two_second<T, Off, SA, SB, 1>(M, O, d, pm, pp, xm, xp, ym, yp, own, mz, k);
     else two_second<T, Off, SA, SB, -1>(M, O, d, pm, pp, xm, xp, ym, yp, own, mz, k);
@@ -331,26 +341,18 @@ struct TwoSrc {
 // Copy the shot's first kMaxSrc2 sources into S (the host launches two-step
 // kernels only for shots with at most that many); returns their count.
 template <typename T>
-__device__ __forceinline__ int two_stage_sources(const Stencil2Args<T>& B, int shot, TwoSrc<T>& S) {
+__device__ __forceinline__ int two_stage_sources(const Stencil2Args<T>& B, int64_t it0, int shot,
+                                                 TwoSrc<T>& S) {
     const int s0 = B.src_start[shot];
     int ns = B.src_start[shot + 1] - s0;
     ns = ns < kMaxSrc2 ? ns : kMaxSrc2;
     const int tid = (int)(threadIdx.y * blockDim.x + threadIdx.x);
     if (tid < ns) {
         S.lin[tid] = B.src_lin[s0 + tid];
-        S.w[tid] = B.src_wav[(int64_t)(s0 + tid) * B.nt + B.it0];
+        S.w[tid] = B.src_wav[(int64_t)(s0 + tid) * B.nt + it0];
     }
     return ns;
 }
-
-// Uniform per-launch values of a wave's march.
-struct TwoGeom {
-    int wy, kbeg, kend, nz, ksrc_lo, ksrc_hi, ns;
-    int64_t col_el, plane;
-    bool in1, own;
-    bool inx;     // the lane's x is its clamped x (pcur there is the lane's own column)
-    bool iny;     // the wave's y is its clamped y
-};
 
 // One march position k: step t+1 at plane k, refill of ring slot U with
 // plane k+PF, the LDS exchange, step t+2 at plane k-1.  GEN = false: k is
@@ -361,7 +363,6 @@ __device__ __forceinline__ void two_iter(const Stencil2Args<T>& B, const March<T
                                          const TwoOut<T, Off>& O, T (*ring)[W][kTileX],
                                          T (*pring)[W][kTileX], const TwoGeom& G,
                                          const TwoSrc<T>& S, TwoCtx<T, PF>& C, int k) {
-    const StencilArgs<T>& A = B.s;
     const int lane = (int)threadIdx.x;
     Mid<T> dn = {};
     const T pp = own_reg(C.qr[U].pp);
@@ -376,7 +377,7 @@ __device__ __forceinline__ void two_iter(const Stencil2Args<T>& B, const March<T
         const T xp = lane == kTileX - 1 ? edge : sr;
         const T ym = G.wy == 0 ? C.qr[U].ym : pring[C.slot][G.wy - 1 < 0 ? 0 : G.wy - 1][lane];
         const T yp = G.wy == W - 1 ? C.qr[U].ym : pring[C.slot][G.wy + 1 > W - 1 ? W - 1 : G.wy + 1][lane];
-        dn = two_first_at<T, Off, SA, SB, ZM>(A, M, k, C.qr[U], C.pz0, pc, xm, xp, ym, yp);
+        dn = two_first_at<T, Off, SA, SB, ZM>(G, M, k, C.qr[U], C.pz0, pc, xm, xp, ym, yp);
         // inject_sources! of step t+1, in source order
         if (k >= G.ksrc_lo && k <= G.ksrc_hi) {
             const int64_t lin = G.col_el + (int64_t)k * G.plane;
@@ -388,7 +389,7 @@ __device__ __forceinline__ void two_iter(const Stencil2Args<T>& B, const March<T
     C.pz0 = C.pz1;
     C.pz1 = pp;
     const int kr = k + PF;
-    if (!GEN || kr <= G.kend) C.qr[U] = two_load<T, Off, W, SA, SB, ZM, GEN>(A, M, kr, G.wy);
+    if (!GEN || kr <= G.kend) C.qr[U] = two_load<T, Off, W, SA, SB, ZM, GEN>(G, M, kr, G.wy);
     __builtin_amdgcn_sched_barrier(0);
     // this column's pcur at plane k+1 for the neighbouring rows (0 for a
     // Dirichlet row), and p^{t+1} at plane k
@@ -406,7 +407,7 @@ __device__ __forceinline__ void two_iter(const Stencil2Args<T>& B, const March<T
             ym = ring[sp][G.wy - 1][lane];
             yp = ring[sp][G.wy + 1][lane];
         }
-        two_second_at<T, Off, SA, SB, ZM>(A, M, O, C.d, C.p1m, dn.p1, xm, xp, ym, yp, G.own, k - 1);
+        two_second_at<T, Off, SA, SB, ZM>(G, M, O, C.d, C.p1m, dn.p1, xm, xp, ym, yp, G.own, k - 1);
     }
     C.p1m = C.d.p1;
     C.d = dn;
@@ -441,14 +442,13 @@ __device__ __forceinline__ void two_march(const Stencil2Args<T>& B, const March<
                                           const TwoOut<T, Off>& O, T (*ring)[W][kTileX],
                                           T (*pring)[W][kTileX], const TwoGeom& G,
                                           const TwoSrc<T>& S) {
-    const StencilArgs<T>& A = B.s;
     TwoCtx<T, PF> C;
     C.pz0 = G.kbeg >= 2 ? ldg(M.pcur, M.col + (Off)(G.kbeg - 2) * M.sm) : T(0);
     C.pz1 = ldg(M.pcur, M.col + (Off)(G.kbeg - 1) * M.sm);
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
         const int kk = G.kbeg - 1 + u;
-        C.qr[u] = kk <= G.kend ? two_load<T, Off, W, SA, SB, ZM, true>(A, M, kk, G.wy) : NodeLoads<T>{};
+        C.qr[u] = kk <= G.kend ? two_load<T, Off, W, SA, SB, ZM, true>(G, M, kk, G.wy) : NodeLoads<T>{};
         __builtin_amdgcn_sched_barrier(0);
     }
     C.d = Mid<T>{};
@@ -475,16 +475,16 @@ __device__ __forceinline__ void two_march(const Stencil2Args<T>& B, const March<
                                                          std::make_integer_sequence<int, PF>{});
 }
 
+// Logical block lb of a launch over one plan; it0 is the 0-based index of
+// step t+1 (the wavelet sample its sources add).
 template <typename T, typename Off, int W, int PF>
-__global__ __launch_bounds__(64 * W) void stencil3d_two(const Stencil2Args<T> B) {
+__device__ __forceinline__ void stencil3d_two_block(const Stencil2Args<T>& B, int64_t it0, int lb) {
     __shared__ T ring[3][W][kTileX];    // p^{t+1} planes
     __shared__ T pring[3][W][kTileX];   // pcur planes
     __shared__ TwoSrc<T> S;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* lds = reinterpret_cast<T*>(smem_raw);
     const StencilArgs<T>& A = B.s;
-    const int lb = logical_block(blockIdx.x, A.grid_blocks);
-    if (lb >= A.logical_blocks) return;  // whole block
 
     // logical order: x tile fastest, then y tile, chunk, shot
     int rest = lb;
@@ -492,7 +492,7 @@ __global__ __launch_bounds__(64 * W) void stencil3d_two(const Stencil2Args<T> B)
     const int ty = rest % A.tiles_y; rest /= A.tiles_y;
     const int chunk = rest % A.nchunks;
     const int shot = rest / A.nchunks;
-    const int ns = two_stage_sources(B, shot, S);
+    const int ns = two_stage_sources(B, it0, shot, S);
     if (A.active[2][0] || A.active[2][1]) stage_march_coef(A, 2, lds);
     __syncthreads();
     const int wy = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
@@ -566,6 +566,9 @@ __global__ __launch_bounds__(64 * W) void stencil3d_two(const Stencil2Args<T> B)
     G.kbeg = kbeg;
     G.kend = kend;
     G.nz = (int)nz;
+    G.halo = (int)halo;
+    G.act[0] = A.active[2][0];
+    G.act[1] = A.active[2][1];
     G.in1 = in1;
     G.own = own;
     G.inx = i == ic;
@@ -702,27 +705,27 @@ __device__ __forceinline__ NodeLoads<T> two2_load_node(const March<T, Off, 2>& M
 }
 
 template <typename T, typename Off, int SA, bool ZM, bool GEN>
-__device__ __forceinline__ NodeLoads<T> two2_load(const StencilArgs<T>& A,
-                                                  const March<T, Off, 2>& M, int k) {
+__device__ __forceinline__ NodeLoads<T> two2_load(const TwoGeom& G, const March<T, Off, 2>& M,
+                                                  int k) {
     int mz;
-    const int zs = ZM ? classify(k, A.ny, A.halo, A.active[1], &mz) : -1;
-    return two2_load_node<T, Off, SA, GEN>(M, (int)A.ny, k, zs);
+    const int zs = ZM ? classify(k, G.nz, G.halo, G.act, &mz) : -1;
+    return two2_load_node<T, Off, SA, GEN>(M, G.nz, k, zs);
 }
 
 template <typename T, typename Off, int SA, bool ZM>
-__device__ __forceinline__ Mid<T> two2_first_at(const StencilArgs<T>& A, const March<T, Off, 2>& M,
+__device__ __forceinline__ Mid<T> two2_first_at(const TwoGeom& G, const March<T, Off, 2>& M,
                                                 int k, const NodeLoads<T>& q, T pm, T pc, T xm,
                                                 T xp) {
     if (!ZM) return two2_first<T, Off, SA, -1>(M, 0, q, pm, pc, xm, xp);
     int mz;
-    const int zs = classify(k, A.ny, A.halo, A.active[1], &mz);
+    const int zs = classify(k, G.nz, G.halo, G.act, &mz);
     if (zs == 0) return two2_first<T, Off, SA, 0>(M, mz, q, pm, pc, xm, xp);
     if (zs == 1) return two2_first<T, Off, SA, 1>(M, mz, q, pm, pc, xm, xp);
     return two2_first<T, Off, SA, -1>(M, mz, q, pm, pc, xm, xp);
 }
 
 template <typename T, typename Off, int SA, bool ZM>
-__device__ __forceinline__ void two2_second_at(const StencilArgs<T>& A, const March<T, Off, 2>& M,
+__device__ __forceinline__ void two2_second_at(const TwoGeom& G, const March<T, Off, 2>& M,
                                                const TwoOut<T, Off>& O, const Mid<T>& d, T pm,
                                                T pp, T xm, T xp, bool own, int k) {
     if (!ZM) {
@@ -730,7 +733,7 @@ __device__ __forceinline__ void two2_second_at(const StencilArgs<T>& A, const Ma
         return;
     }
     int mz;
-    const int zs = classify(k, A.ny, A.halo, A.active[1], &mz);
+    const int zs = classify(k, G.nz, G.halo, G.act, &mz);
     if (zs == 0) two2_second<T, Off, SA, 0>(M, O, d, pm, pp, xm, xp, own, mz, k);
     else if (zs == 1) two2_second<T, Off, SA, 1>(M, O, d, pm, pp, xm, xp, own, mz, k);
     else two2_second<T, Off, SA, -1>(M, O, d, pm, pp, xm, xp, own, mz, k);
@@ -741,7 +744,6 @@ template <typename T, typename Off, int PF, int SA, bool ZM, bool GEN, int U>
 __device__ __forceinline__ void two2_iter(const Stencil2Args<T>& B, const March<T, Off, 2>& M,
                                           const TwoOut<T, Off>& O, const TwoGeom& G,
                                           const TwoSrc<T>& S, TwoCtx<T, PF>& C, int k) {
-    const StencilArgs<T>& A = B.s;
     const int lane = (int)threadIdx.x;
     Mid<T> dn = {};
     const T pp = own_reg(C.qr[U].pp);
@@ -752,7 +754,7 @@ __device__ __forceinline__ void two2_iter(const Stencil2Args<T>& B, const March<
         const T edge = C.qr[U].xm;
         const T xm = lane == 0 ? edge : sl;
         const T xp = lane == kTileX - 1 ? edge : sr;
-        dn = two2_first_at<T, Off, SA, ZM>(A, M, k, C.qr[U], C.pz0, pc, xm, xp);
+        dn = two2_first_at<T, Off, SA, ZM>(G, M, k, C.qr[U], C.pz0, pc, xm, xp);
         // inject_sources! of step t+1, in source order
         if (k >= G.ksrc_lo && k <= G.ksrc_hi) {
             const int64_t lin = G.col_el + (int64_t)k * G.plane;
@@ -764,14 +766,14 @@ __device__ __forceinline__ void two2_iter(const Stencil2Args<T>& B, const March<
     C.pz0 = C.pz1;
     C.pz1 = pp;
     const int kr = k + PF;
-    if (!GEN || kr <= G.kend) C.qr[U] = two2_load<T, Off, SA, ZM, GEN>(A, M, kr);
+    if (!GEN || kr <= G.kend) C.qr[U] = two2_load<T, Off, SA, ZM, GEN>(G, M, kr);
     __builtin_amdgcn_sched_barrier(0);
     if (G.own && (!GEN || (k >= G.kbeg && k < G.kend)))
         stg_stream(O.p1, M.col + (Off)k * M.sm, dn.p1);
     if (!GEN || k - 1 >= G.kbeg) {
         const T xm = lane_shift(C.d.p1, true);
         const T xp = lane_shift(C.d.p1, false);
-        two2_second_at<T, Off, SA, ZM>(A, M, O, C.d, C.p1m, dn.p1, xm, xp, G.own, k - 1);
+        two2_second_at<T, Off, SA, ZM>(G, M, O, C.d, C.p1m, dn.p1, xm, xp, G.own, k - 1);
     }
     C.p1m = C.d.p1;
     C.d = dn;
@@ -793,14 +795,13 @@ template <typename T, typename Off, int PF, int SA, bool ZM>
 __device__ __forceinline__ void two2_march(const Stencil2Args<T>& B, const March<T, Off, 2>& M,
                                            const TwoOut<T, Off>& O, const TwoGeom& G,
                                            const TwoSrc<T>& S) {
-    const StencilArgs<T>& A = B.s;
     TwoCtx<T, PF> C;
     C.pz0 = G.kbeg >= 2 ? ldg(M.pcur, M.col + (Off)(G.kbeg - 2) * M.sm) : T(0);
     C.pz1 = ldg(M.pcur, M.col + (Off)(G.kbeg - 1) * M.sm);
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
         const int kk = G.kbeg - 1 + u;
-        C.qr[u] = kk <= G.kend ? two2_load<T, Off, SA, ZM, true>(A, M, kk) : NodeLoads<T>{};
+        C.qr[u] = kk <= G.kend ? two2_load<T, Off, SA, ZM, true>(G, M, kk) : NodeLoads<T>{};
         __builtin_amdgcn_sched_barrier(0);
     }
     C.d = Mid<T>{};
@@ -820,20 +821,18 @@ __device__ __forceinline__ void two2_march(const Stencil2Args<T>& B, const March
 }
 
 template <typename T, typename Off, int BY, int PF>
-__global__ __launch_bounds__(64 * BY) void stencil2d_two(const Stencil2Args<T> B) {
+__device__ __forceinline__ void stencil2d_two_block(const Stencil2Args<T>& B, int64_t it0, int lb) {
     __shared__ TwoSrc<T> S;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* lds = reinterpret_cast<T*>(smem_raw);
     const StencilArgs<T>& A = B.s;
-    const int lb = logical_block(blockIdx.x, A.grid_blocks);
-    if (lb >= A.logical_blocks) return;  // whole block
 
     // logical order: x tile fastest, then chunk group, shot
     int rest = lb;
     const int tx = rest % A.tiles_x; rest /= A.tiles_x;
     const int ty = rest % A.tiles_y;
     const int shot = rest / A.tiles_y;
-    const int ns = two_stage_sources(B, shot, S);
+    const int ns = two_stage_sources(B, it0, shot, S);
     if (A.active[1][0] || A.active[1][1]) stage_march_coef(A, 1, lds);
     __syncthreads();
     const int wy = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
@@ -904,6 +903,9 @@ __global__ __launch_bounds__(64 * BY) void stencil2d_two(const Stencil2Args<T> B
     G.kbeg = kbeg;
     G.kend = kend;
     G.nz = (int)ny;
+    G.halo = (int)halo;
+    G.act[0] = A.active[1][0];
+    G.act[1] = A.active[1][1];
     G.in1 = in1;
     G.own = own;
     G.inx = in1;
@@ -918,6 +920,52 @@ __global__ __launch_bounds__(64 * BY) void stencil2d_two(const Stencil2Args<T> B
         if (zm) two2_march<T, Off, PF, 0, true>(B, M, O, G, S);
         else two2_march<T, Off, PF, 0, false>(B, M, O, G, S);
     }
+}
+
+// ---------------------------------------------------------------------
+// Entry kernels.  A plain launch covers one plan.  A batch launch covers up
+// to kMaxBatch plans (different grids, same precision) in one grid, as the
+// single-step batch kernel does: logical block b belongs to the plan p with
+// block_start[p] <= b < block_start[p+1], whose arguments are table[p].
+// Every plan of a batch launch runs both steps.
+// ---------------------------------------------------------------------
+template <typename T>
+struct Batch2Args {
+    const Stencil2Args<T>* table;  // per-plan arguments for this launch's buffer roles
+    int64_t it0;                   // 0-based index of step t+1
+    int block_start[kMaxBatch + 1];
+    int nplans;
+    int grid_blocks;
+};
+
+template <typename T, typename Off, int W, int PF>
+__global__ __launch_bounds__(64 * W) void stencil3d_two(const Stencil2Args<T> B) {
+    const int lb = logical_block(blockIdx.x, B.s.grid_blocks);
+    if (lb >= B.s.logical_blocks) return;  // whole block
+    stencil3d_two_block<T, Off, W, PF>(B, B.it0, lb);
+}
+
+template <typename T, typename Off, int BY, int PF>
+__global__ __launch_bounds__(64 * BY) void stencil2d_two(const Stencil2Args<T> B) {
+    const int lb = logical_block(blockIdx.x, B.s.grid_blocks);
+    if (lb >= B.s.logical_blocks) return;  // whole block
+    stencil2d_two_block<T, Off, BY, PF>(B, B.it0, lb);
+}
+
+template <typename T, typename Off, int W, int PF>
+__global__ __launch_bounds__(64 * W) void stencil3d_two_batch(const Batch2Args<T> B) {
+    const int lb = logical_block(blockIdx.x, B.grid_blocks);
+    if (lb >= B.block_start[B.nplans]) return;  // whole block
+    const int p = batch_plan(B.block_start, B.nplans, lb);
+    stencil3d_two_block<T, Off, W, PF>(B.table[p], B.it0, lb - B.block_start[p]);
+}
+
+template <typename T, typename Off, int BY, int PF>
+__global__ __launch_bounds__(64 * BY) void stencil2d_two_batch(const Batch2Args<T> B) {
+    const int lb = logical_block(blockIdx.x, B.grid_blocks);
+    if (lb >= B.block_start[B.nplans]) return;  // whole block
+    const int p = batch_plan(B.block_start, B.nplans, lb);
+    stencil2d_two_block<T, Off, BY, PF>(B.table[p], B.it0, lb - B.block_start[p]);
 }
 
 }  // namespace awcpml

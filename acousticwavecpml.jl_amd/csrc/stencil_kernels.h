@@ -101,15 +101,22 @@ __device__ __forceinline__ int logical_block(int b, int grid) {
     return (b % kNumXcd) * per + b / kNumXcd;
 }
 
-// Loads/stores at a byte offset from a base pointer.  With a uniform base
-// and a 32-bit offset the compiler emits the scalar-base addressing form.
+// Loads/stores of global memory at a byte offset from a base pointer.
+// With a uniform base and a 32-bit offset the compiler emits the
+// scalar-base addressing form.  The access goes through a global
+// (address space 1) pointer: a base read from a device-side argument table
+// is otherwise generic to the compiler, whose flat_* accesses also count
+// against the LDS wait counter and serialise the march's waits.
+template <typename T>
+using GlobalPtr = __attribute__((address_space(1))) T*;
+
 template <typename T, typename Off>
 __device__ __forceinline__ T ldg(const T* base, Off byte_off) {
-    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+    return *(GlobalPtr<const T>)((const char*)base + byte_off);
 }
 template <typename T, typename Off>
 __device__ __forceinline__ void stg(T* base, Off byte_off, T v) {
-    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off) = v;
+    *(GlobalPtr<T>)((char*)base + byte_off) = v;
 }
 // Streaming variants for data touched once per launch (pold is read once,
 // pnew is written once and next read in the following launch): the
@@ -117,12 +124,11 @@ __device__ __forceinline__ void stg(T* base, Off byte_off, T v) {
 // lines in L2.
 template <typename T, typename Off>
 __device__ __forceinline__ T ldg_stream(const T* base, Off byte_off) {
-    return __builtin_nontemporal_load(
-        reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off));
+    return __builtin_nontemporal_load((GlobalPtr<const T>)((const char*)base + byte_off));
 }
 template <typename T, typename Off>
 __device__ __forceinline__ void stg_stream(T* base, Off byte_off, T v) {
-    __builtin_nontemporal_store(v, reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off));
+    __builtin_nontemporal_store(v, (GlobalPtr<T>)((char*)base + byte_off));
 }
 
 // Classify a node along one axis: side 0/1 and layer m, or side -1.

@@ -173,8 +173,8 @@ int awcpml_get_shot_group(const awcpml_plan *plan, int *shot_group);
  * first call with steps = 2 allocates one more field buffer and one more
  * psi and xi buffer per CPML side.  awcpml_get_time_block returns the
  * steps a launch advances now (1 or 2) and the effective block_rows,
- * march_len and prefetch.  A plan that ran two-step launches is reset before it joins a
- * batch.  No reference counterpart (the reference launches one kernel per
+ * march_len and prefetch.  A plan that ran two-step launches on its own is
+ * reset before it joins a batch.  No reference counterpart (the reference launches one kernel per
  * time step, acoustic_2D_xPU.jl:327-345, acoustic_3D_xPU.jl:392-407). */
 int awcpml_set_time_block(awcpml_plan *plan, int steps, int block_rows, int march_len,
                           int prefetch);
@@ -201,7 +201,8 @@ int awcpml_set_graph(awcpml_plan *plan, int enable);
 
 /* Group 1..32 plans that share ndim, dtype and tiling (grids, halos,
  * models, shot counts and nt may all differ) so that every time step of
- * all of them is one stencil launch and one source/receiver launch.  The
+ * all of them is one stencil launch and one source/receiver launch (or every
+ * two steps, see awcpml_batch_set_time_block).  The
  * plans stay owned by the caller and must be fully configured; a plan whose
  * nt is shorter than the batch's longest stops at its own nt.  Results are
  * read per plan with awcpml_get_traces / awcpml_get_field.  Extension
@@ -221,6 +222,28 @@ int awcpml_batch_time_advance(awcpml_batch *batch, int64_t nsteps, double *ms_to
 /* Time nlaunches batch stencil launches alone (ms); the plans' wavefields
  * advance without sources; call awcpml_batch_reset afterwards. */
 int awcpml_batch_time_stencil(awcpml_batch *batch, int nlaunches, double *ms_total);
+
+/* Time blocking for the whole batch: with steps = 2, each pair of steps of
+ * every plan is one stencil launch (and two source/receiver launches) while
+ * every plan of the batch has two steps left and two-step launches apply
+ * to all of them (see awcpml_set_time_block); single steps otherwise.
+ * block_rows (3D 4, 8 or 16; 2D 4 or 8) and prefetch (1..3; 0: the
+ * library's choice) apply to the whole launch; each plan keeps its own
+ * march_len.  Calls awcpml_set_time_block(plan, steps, block_rows,
+ * <its march_len>, prefetch) on every plan, so the first call with steps = 2
+ * allocates the plans' spare buffers.  Results are bit-identical to single
+ * steps.  A batch starts with steps = 2 when all its plans have steps = 2
+ * with the same block_rows and prefetch.  After a batch run the plans'
+ * buffers may have traded roles; the plans read their results as usual.  awcpml_batch_get_time_block returns
+ * the steps a launch advances now and the effective block_rows and
+ * prefetch; awcpml_batch_time_stencil2 times nlaunches two-step launches
+ * alone (ms; call awcpml_batch_reset afterwards).  No reference
+ * counterpart (acoustic_3D_xPU.jl:392-407 launches one kernel per step of
+ * one model). */
+int awcpml_batch_set_time_block(awcpml_batch *batch, int steps, int block_rows, int prefetch);
+int awcpml_batch_get_time_block(const awcpml_batch *batch, int *steps, int *block_rows,
+                                int *prefetch);
+int awcpml_batch_time_stencil2(awcpml_batch *batch, int nlaunches, double *ms_total);
 
 /* ------------------------------------------------------------------ */
 /* Differentiation: the FWI gradient by a hand-written adjoint.        */

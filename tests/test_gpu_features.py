@@ -10,6 +10,7 @@ import pytest
 
 import awcpml
 from awcpml import _lib
+from awcpml.autotune import two_step_batch_candidates
 from cases import oracle_problem, rel_l2, small2d, small3d
 
 pytestmark = pytest.mark.gpu
@@ -233,7 +234,7 @@ def test_autotune_batch_keeps_results():
     oracles = [oracle_for(c) for c in cases]
     plans = [plan_from_oracle(o) for o in oracles]
     best = awcpml.autotune_batch(plans, launches=1)
-    assert best["tried"] == 18
+    assert best["tried"] == 18 + len(two_step_batch_candidates(plans))
     assert all(P.tiling() == (best["block_y"], best["march_len"], best["prefetch"]) for P in plans)
     B = awcpml.Batch(plans)
     B.run()

@@ -260,3 +260,160 @@ def test_two_step_fwi_gradient_unchanged():
     assert J1 == J2
     assert np.array_equal(g1, g2)
     P.close()
+
+
+# ---------------------------------------------------------------------
+# Batches of plans, two steps of every plan per launch
+# (awcpml_batch_set_time_block): bit-identical to single-step batches.
+
+BATCH_3D = [dict(n=(37, 29, 23), halo=5, nt=70, seed=1),
+            dict(n=(45, 41, 33), halo=6, nt=95, seed=2, freetop=False, nsrc=2),
+            dict(n=(29, 31, 27), halo=4, nt=50, seed=3)]
+
+
+def _batch_state(plans):
+    out = []
+    for P in plans:
+        out += _state(P, P.nshots)
+    return out
+
+
+def _batch_run(B, nt, splits=None):
+    B.reset()
+    done = 0
+    for n in (splits or ()):
+        B.advance(n)
+        done += n
+    B.advance(nt - done)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_batch_two_step_bitwise(dtype):
+    print("[gpu] batch two-step 3D %s" % dtype, flush=True)
+    oracles = [_oracle(small3d(**c)) for c in BATCH_3D]
+    plans = [_plan(o, dtype=dtype) for o in oracles]
+    nt = max(o.nt for o in oracles)
+    B = awcpml.Batch(plans)
+    assert B.time_block()[0] == 1
+    B.run()
+    one = _batch_state(plans)
+    B.close()
+    # each plan keeps its own chunk length
+    for P, m in zip(plans, (0, 5, 7)):
+        P.set_time_block(1, 0, m)
+    for rows, pf in ((16, 2), (8, 1), (4, 3)):
+        print("[gpu]   rows %d prefetch %d" % (rows, pf), flush=True)
+        for P in plans:
+            P.reset()  # a batch run may leave the buffers in other roles
+        B = awcpml.Batch(plans)
+        B.set_time_block(2, rows, pf)
+        assert B.time_block() == (2, rows, pf)
+        for graph in (False, True):
+            B.set_graph(graph)
+            for splits in (None, (3, 8), (1,)):
+                _batch_run(B, nt, splits)
+                _same(_batch_state(plans), one)
+        B.close()
+    if dtype == "f64":
+        for P, o in zip(plans, oracles):
+            o.run()
+            assert np.array_equal(P.traces(), o.traces())
+            assert np.array_equal(P.field(), o.pend())
+    for P in plans:
+        P.close()
+
+
+def test_batch_two_step_inherits_and_replays():
+    print("[gpu] batch two-step: setting taken from the plans, graph replays", flush=True)
+    oracles = [_oracle(small3d(**c)) for c in BATCH_3D[:2]]
+    plans = [_plan(o, dtype="f32") for o in oracles]
+    nt = max(o.nt for o in oracles)
+    B = awcpml.Batch(plans)
+    B.run()
+    one = _batch_state(plans)
+    B.close()
+    for P in plans:
+        P.set_time_block(2, 8, 0, 2)
+        P.reset()
+    B = awcpml.Batch(plans)
+    assert B.time_block() == (2, 8, 2)
+    B.set_graph(True)
+    for rep in range(3):
+        _batch_run(B, nt, (4,))
+        _same(_batch_state(plans), one)
+        # the plans read and run on their own between replays
+        assert np.array_equal(plans[0].traces(), one[0][0])
+    B.close()
+    # after the batch is gone the plans run alone as before
+    for P, o in zip(plans, oracles):
+        P.reset()
+        P.advance(o.nt)
+    _same(_batch_state(plans), one)
+    for P in plans:
+        P.close()
+
+
+def test_batch_two_step_2d_bitwise():
+    print("[gpu] batch two-step 2D fp64, one plan with 2 shots", flush=True)
+    oracles = [_oracle(_interior_2d(nt=61)),
+               _oracle(_interior_2d(n=(150, 47), halo=7, freetop=False, nt=44))]
+    plans = [_plan(oracles[0], nshots=2), _plan(oracles[1])]
+    nt = max(o.nt for o in oracles)
+    B = awcpml.Batch(plans)
+    B.run()
+    one = _batch_state(plans)
+    B.close()
+    for rows, pf in ((4, 2), (8, 3)):
+        for P in plans:
+            P.reset()  # a batch run may leave the buffers in other roles
+        B = awcpml.Batch(plans)
+        B.set_time_block(2, rows, pf)
+        assert B.time_block()[0] == 2
+        for graph in (False, True):
+            B.set_graph(graph)
+            _batch_run(B, nt, (5,))
+            _same(_batch_state(plans), one)
+        B.close()
+    for o in oracles:
+        o.run()
+    assert np.array_equal(plans[0].traces(1), oracles[0].traces())
+    assert np.array_equal(plans[1].traces(), oracles[1].traces())
+    for P in plans:
+        P.close()
+
+
+def test_batch_two_step_falls_back_for_dirichlet_source():
+    print("[gpu] batch two-step fallback: a plan with a source on the top row", flush=True)
+    oracles = [_oracle(small2d(nt=40)), _oracle(_interior_2d(nt=40))]
+    plans = [_plan(o) for o in oracles]
+    B = awcpml.Batch(plans)
+    B.set_time_block(2, 4, 2)
+    assert B.time_block()[0] == 1
+    B.run()
+    for P, o in zip(plans, oracles):
+        o.run()
+        assert np.array_equal(P.traces(), o.traces())
+    B.close()
+    for P in plans:
+        P.close()
+
+
+def test_autotune_batch_two_step_keeps_results():
+    print("[gpu] autotune_batch with two-step candidates", flush=True)
+    oracles = [_oracle(small3d(**c)) for c in BATCH_3D]
+    plans = [_plan(o, dtype="f32") for o in oracles]
+    nt = max(o.nt for o in oracles)
+    B = awcpml.Batch(plans)
+    B.run()
+    one = _batch_state(plans)
+    B.close()
+    for P in plans:
+        P.reset()
+    best = awcpml.autotune_batch(plans, cands=[(4, 32, 2)], launches=2)
+    B = awcpml.Batch(plans)
+    assert B.time_block()[0] == (2 if best["time_block"] else 1)
+    _batch_run(B, nt)
+    _same(_batch_state(plans), one)
+    B.close()
+    for P in plans:
+        P.close()
