@@ -56,12 +56,13 @@ def two_step_candidates(plan):
 
 def two_step_batch_candidates(plans):
     """(block_rows, march_len, prefetch) of two-step batch launches worth
-    timing; march_len is applied to every plan."""
+    timing; march_len is applied to every plan (0: each plan's default,
+    sized to its grid)."""
     if plans[0].ndim == 2:
-        return [(w, ml, pf) for w in (4, 8) for ml in (32, 64) for pf in (1, 2)]
+        return [(w, ml, pf) for w in (4, 8) for ml in (0, 32, 64) for pf in (1, 2)]
     if plans[0].dtype == 0:
-        return [(8, ml, pf) for ml in (24, 48) for pf in (1, 2)] + [(4, 24, 2)]
-    return ([(16, ml, pf) for ml in (24, 48, 96, 144) for pf in (1, 2)]
+        return [(8, ml, pf) for ml in (0, 24, 48) for pf in (1, 2)] + [(4, 24, 2)]
+    return ([(16, ml, pf) for ml in (0, 24, 48, 96, 144) for pf in (1, 2)]
             + [(8, ml, 2) for ml in (24, 48)])
 
 
