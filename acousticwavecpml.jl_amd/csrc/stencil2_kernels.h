@@ -398,15 +398,15 @@ __device__ __forceinline__ void two_iter(const Stencil2Args<T>& B, const March<T
     if (G.own && (!GEN || (k >= G.kbeg && k < G.kend)))
         stg_stream(O.p1, M.col + (Off)k * M.sm, dn.p1);
     __syncthreads();
-    if (!GEN || k - 1 >= G.kbeg) {
+    // step t+2 on the owned rows only: the first and last wave are the
+    // block's halo rows, which store nothing (a wave-uniform branch that
+    // leaves their issue slots to the other waves)
+    if ((!GEN || k - 1 >= G.kbeg) && G.wy >= 1 && G.wy <= W - 2) {
         const int sp = C.slot == 0 ? 2 : C.slot - 1;  // plane k-1
         const T xm = lane_shift(C.d.p1, true);
         const T xp = lane_shift(C.d.p1, false);
-        T ym = T(0), yp = T(0);
-        if (G.wy >= 1 && G.wy <= W - 2) {
-            ym = ring[sp][G.wy - 1][lane];
-            yp = ring[sp][G.wy + 1][lane];
-        }
+        const T ym = ring[sp][G.wy - 1][lane];
+        const T yp = ring[sp][G.wy + 1][lane];
         two_second_at<T, Off, SA, SB, ZM>(G, M, O, C.d, C.p1m, dn.p1, xm, xp, ym, yp, G.own, k - 1);
     }
     C.p1m = C.d.p1;
