@@ -417,3 +417,41 @@ def test_autotune_batch_two_step_keeps_results():
     B.close()
     for P in plans:
         P.close()
+
+
+def test_time_advance_equals_advance():
+    print("[gpu] time_advance (plan and batch, one and two steps, graphs) equals advance",
+          flush=True)
+    oracles = [_oracle(small3d(**c)) for c in BATCH_3D[:2]]
+    plans = [_plan(o, dtype="f32") for o in oracles]
+    nt = max(o.nt for o in oracles)
+    B = awcpml.Batch(plans)
+    B.run()
+    one = _batch_state(plans)
+    B.close()
+    for steps in (1, 2):
+        for graph in (False, True):
+            for P in plans:
+                P.reset()
+            B = awcpml.Batch(plans)
+            B.set_time_block(steps, 8, 2)
+            B.set_graph(graph)
+            B.reset()
+            ms = B.time_advance(3)
+            ms += B.time_advance(nt - 3)
+            assert ms > 0.0
+            _same(_batch_state(plans), one)
+            B.close()
+    # a plan alone, with its own time blocking and graphs
+    P, o = plans[0], oracles[0]
+    for steps in (1, 2):
+        P.set_time_block(steps, 16, 0, 2)
+        P.set_graph(True)
+        P.reset()
+        P.time_advance(5)
+        P.time_advance(o.nt - 5)
+        assert np.array_equal(P.traces(), one[0][0])
+        assert np.array_equal(P.field(0, 1), one[0][2])
+    P.set_graph(False)
+    for P in plans:
+        P.close()
