@@ -455,3 +455,28 @@ def test_time_advance_equals_advance():
     P.set_graph(False)
     for P in plans:
         P.close()
+
+
+@pytest.mark.parametrize("nts", [(1, 7), (2, 3), (5,)])
+def test_batch_two_step_short_runs(nts):
+    print("[gpu] batch two-step, short runs nt=%s" % (nts,), flush=True)
+    oracles = [_oracle(small3d(**dict(BATCH_3D[q], nt=n))) for q, n in enumerate(nts)]
+    plans = [_plan(o) for o in oracles]
+    nt = max(nts)
+    B = awcpml.Batch(plans)
+    B.run()
+    one = _batch_state(plans)
+    B.close()
+    for P in plans:
+        P.reset()
+    B = awcpml.Batch(plans)
+    B.set_time_block(2, 8, 2)
+    for graph in (False, True):
+        B.set_graph(graph)
+        _batch_run(B, nt)
+        _same(_batch_state(plans), one)
+    B.close()
+    for P, o in zip(plans, oracles):
+        o.run()
+        assert np.array_equal(P.traces(), o.traces())
+        P.close()
