@@ -1621,7 +1621,9 @@ int awcpml_batch_create(awcpml_plan* const* plans, int count, awcpml_batch** out
         int st = ready(plans[q]);
         if (st) return st;
         if (!at_home(plans[q]))
-            return fail(AWCPML_ERR_STATE, "plan %d ran two-step launches: reset it before batching", q);
+            return fail(AWCPML_ERR_STATE,
+                        "plan %d's buffers traded roles in two-step launches (alone or in a batch) "
+                        "since its last reset: reset it before batching", q);
         if (plans[q]->block_y != plans[0]->block_y || plans[q]->prefetch != plans[0]->prefetch)
             return fail(AWCPML_ERR_ARG, "plans of a batch must share block_y and prefetch (plan %d differs)", q);
     }

@@ -323,6 +323,24 @@ def test_batch_two_step_bitwise(dtype):
         P.close()
 
 
+def test_batch_of_plans_in_traded_roles_is_refused():
+    print("[gpu] a plan whose buffers traded roles must be reset before batching", flush=True)
+    oracles = [_oracle(small3d(**c)) for c in BATCH_3D[:2]]
+    plans = [_plan(o, dtype="f32") for o in oracles]
+    B = awcpml.Batch(plans)
+    B.set_time_block(2, 8, 2)
+    B.reset()
+    B.advance(2)  # one two-step launch: the spares now hold step 2
+    B.close()
+    with pytest.raises(awcpml._lib.AwcpmlError, match="reset it before batching"):
+        awcpml.Batch(plans)
+    for P in plans:
+        P.reset()
+    awcpml.Batch(plans).close()
+    for P in plans:
+        P.close()
+
+
 def test_batch_two_step_inherits_and_replays():
     print("[gpu] batch two-step: setting taken from the plans, graph replays", flush=True)
     oracles = [_oracle(small3d(**c)) for c in BATCH_3D[:2]]
