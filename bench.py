@@ -352,6 +352,45 @@ def profile_kernel(S, dtype, nshots, tiling, launches, time_block=None):
     print(f"[profile] calibration copy: 3 x {n * 4} bytes read + written", flush=True)
 
 
+def profile_uneven(args):
+    """Profiler target: only the batch stencil launches of the uneven 3D
+    workload at a fixed launch shape (no autotuning)."""
+    import awcpml
+    from cases import lt_for_nt, uneven3d
+    plans, setups = [], []
+    for c in uneven3d():
+        vel = c["make_vel"]()
+        c = dict(c)
+        c["lt"] = lt_for_nt(3, (10.0,) * 3, float(vel.max()), c["nt"])
+        S = _setup(c, vel=vel)
+        del vel
+        plans.append(awcpml.Plan.from_setup(S, dtype="f32"))
+        setups.append(S)
+    tiling = tuple(int(v) for v in args.tiling.split(",")) if args.tiling else (4, 64, 2)
+    for P in plans:
+        P.set_tiling(*tiling[:3])
+    B = awcpml.Batch(plans)
+    steps = 1
+    if args.time_block:
+        w, ml, pf = (int(v) for v in args.time_block.split(","))
+        for P in plans:
+            P.set_time_block(1, 0, ml)
+        B.set_time_block(2, w, pf)
+        steps = 2
+    B.reset()
+    n = args.launches
+    timer = B.time_stencil2 if steps == 2 else B.time_stencil
+    timer(1)  # loads the kernel's code object outside the timed launches
+    ms = timer(n) / n
+    bytes_step = sum(algorithmic_bytes(S.n, S.halo, False, 4, 1)[0] for S in setups)
+    print(f"[profile] uneven3d batch FP32 time_block={B.time_block()} {n} launches "
+          f"{ms:.4f} ms/launch, {bytes_step * steps} algorithmic bytes/launch, "
+          f"{bytes_step * steps / ms / 1e6:.1f} GB/s", flush=True)
+    B.close()
+    for P in plans:
+        P.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -373,6 +412,10 @@ def main():
     ap.add_argument("--launches", type=int, default=20)
     ap.add_argument("--profile-fwi", action="store_true",
                     help="profiler target: one FP32 FWI gradient of the 554^2 32-shot workload")
+    ap.add_argument("--profile-uneven", action="store_true",
+                    help="profiler target: batch stencil launches of the 12-model uneven 3D "
+                         "workload (FP32); --time-block rows,march_len,prefetch for two-step "
+                         "launches, --tiling block_y,march_len,prefetch for single steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -398,6 +441,10 @@ def main():
         J, g = fwi.misfit_gradient(P, obs, every=fc["every"])
         print(f"[profile] fwi2d FP32 gradient {time.perf_counter() - t0:.3f} s, misfit {J:.4e}", flush=True)
         P.close()
+        return
+
+    if args.profile_uneven:
+        profile_uneven(args)
         return
 
     case = plain3d(nt=args.nt)
