@@ -55,6 +55,7 @@ SIGNATURES = {
     "awcpml_get_time_block": (ctypes.c_int, [_vp] + [ctypes.POINTER(ctypes.c_int)] * 4),
     "awcpml_time_stencil2": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "awcpml_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "awcpml_set_wide_offsets": (ctypes.c_int, [_vp, ctypes.c_int]),
     "awcpml_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(_vp)]),
     "awcpml_batch_destroy": (ctypes.c_int, [_vp]),
     "awcpml_batch_reset": (ctypes.c_int, [_vp]),
@@ -67,6 +68,7 @@ SIGNATURES = {
     "awcpml_batch_time_stencil2": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "awcpml_checkpoint_run": (ctypes.c_int, [_vp, ctypes.c_int64]),
     "awcpml_adjoint": (ctypes.c_int, [_vp, _dp, ctypes.c_int64, _dp]),
+    "awcpml_set_gradient_fold": (ctypes.c_int, [_vp, ctypes.c_int64]),
     "awcpml_last_error": (ctypes.c_char_p, []),
     "awcpml_version": (ctypes.c_char_p, []),
 }

@@ -46,6 +46,9 @@ class Problem:
         self.S = S
         self.dt = S.dt
         self.every = every
+        # number of checkpointed forward runs so far: a backward call checks
+        # that the plan still holds the checkpoints of its own forward
+        self.forward_count = 0
         self.plan = Plan(S.n, S.halo, dtype=dtype, nshots=nshots)
         self.plan.set_model(S.fact, S.inv_d, S.inv_d2)
         for a in range(S.ndim):
@@ -60,6 +63,28 @@ class Problem:
         v = np.asarray(vel, dtype=np.float64)
         fact = np.asfortranarray((self.dt * self.dt) * (v * v))
         self.plan.set_model(fact, self.S.inv_d, self.S.inv_d2)
+
+    def forward(self, vel):
+        """Checkpointed forward run for ``vel``; returns the traces
+        (nshots, nt, nrecs) and the run's number, which :meth:`pullback`
+        needs."""
+        self.set_velocity(vel)
+        P = self.plan
+        P.checkpoint_run(self.every)
+        self.forward_count += 1
+        return np.stack([P.traces(s) for s in range(P.nshots)]), self.forward_count
+
+    def pullback(self, run, trace_bar):
+        """dJ/dfact for the cotangents ``trace_bar`` of forward run ``run``.
+        Raises if another forward ran on this problem since: the plan's
+        checkpoints and model then belong to that run, and the adjoint would
+        mix the two models."""
+        if run != self.forward_count:
+            raise RuntimeError(
+                f"stale backward: forward run {run} was followed by run {self.forward_count} on "
+                "the same Problem; the checkpoints of run %d are gone (call backward before the "
+                "next forward, or use one Problem per pending backward)" % run)
+        return self.plan.adjoint(trace_bar)
 
     def close(self):
         self.plan.close()
@@ -76,12 +101,9 @@ def _function():
 
         @staticmethod
         def forward(ctx, vel, problem):
-            v = vel.detach().cpu().numpy()
-            problem.set_velocity(v)
-            P = problem.plan
-            P.checkpoint_run(problem.every)
-            tr = np.stack([P.traces(s) for s in range(P.nshots)])
+            tr, run = problem.forward(vel.detach().cpu().numpy())
             ctx.problem = problem
+            ctx.run = run
             ctx.save_for_backward(vel)
             return torch.from_numpy(tr)
 
@@ -90,7 +112,7 @@ def _function():
             (vel,) = ctx.saved_tensors
             problem = ctx.problem
             tb = tbar.detach().cpu().numpy().astype(np.float64)
-            gf = problem.plan.adjoint([tb[s] for s in range(tb.shape[0])])
+            gf = problem.pullback(ctx.run, [tb[s] for s in range(tb.shape[0])])
             gv = velocity_gradient(gf, vel.detach().cpu().numpy(), problem.dt)
             return torch.from_numpy(gv).to(vel.dtype), None
 
@@ -105,7 +127,8 @@ def solve_traces(vel, problem):
 
     The checkpoints of the forward call are reused by the backward call, so
     each forward must be followed by its backward before the next forward
-    on the same problem."""
+    on the same problem; a backward after a later forward raises
+    RuntimeError instead of returning a gradient of the wrong model."""
     global _SolveTraces
     if _SolveTraces is None:
         _SolveTraces = _function()

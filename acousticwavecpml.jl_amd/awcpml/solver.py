@@ -102,6 +102,11 @@ class Plan:
         """Capture the time loop in HIP graphs (replayed on later solves)."""
         _lib.check(_lib.load().awcpml_set_graph(self._h, int(bool(enable))))
 
+    def set_wide_offsets(self, enable=True):
+        """Use the 64-bit-offset kernels (otherwise only for fields of
+        4 GiB and more per shot); bit-identical results."""
+        _lib.check(_lib.load().awcpml_set_wide_offsets(self._h, int(bool(enable))))
+
     def set_tiling(self, block_y=0, march_len=0, prefetch=0):
         _lib.check(_lib.load().awcpml_set_tiling(self._h, block_y, march_len, prefetch))
 
@@ -158,6 +163,12 @@ class Plan:
         """Reset, run all nt steps and keep a forward checkpoint every
         ``every`` steps (0: round(sqrt(nt))) for :meth:`adjoint`."""
         _lib.check(_lib.load().awcpml_checkpoint_run(self._h, int(every)))
+
+    def set_gradient_fold(self, steps=0):
+        """Reverse steps summed in the plan's precision before they are
+        folded into the FP64 gradient total (0: one checkpoint segment, the
+        default; -1: never)."""
+        _lib.check(_lib.load().awcpml_set_gradient_fold(self._h, int(steps)))
 
     def adjoint(self, trace_bar):
         """dJ/dfact summed over the shots, for the trace cotangents
@@ -345,14 +356,22 @@ def _solve(lengths, lt, vel, srcs, recs, halo, rcoef, ppw, freetop, do_bench, do
 
 
 def _save_snapshot(save_dir, save_name, it, lengths, S, pcur):
-    """do_save snapshot with the fields of acoustic_3D_xPU.jl:480-498
-    (lx, ly, lz, halo, pcur, possrcs, posrecs), written as .npz because no
-    HDF5 library is available here."""
+    """do_save snapshot of step ``it`` (acoustic_3D_xPU.jl:480-498): the
+    reference's seven datasets with its names, shapes, element types and
+    index order -- lx, ly, lz (Float64 scalars), halo (Int64 scalar), pcur
+    (Float64, nx x ny x nz, indexed [i, j, k]), possrcs and posrecs (Int64,
+    count x 3, 1-based grid positions) -- in ``<save_name>_it<it>.npz``
+    instead of the reference's ``.h5`` (no HDF5 library in this Python
+    environment; the Julia backend writes the HDF5 file itself).  An
+    existing file of that name is replaced, as in the reference."""
     path = os.path.join(save_dir, f"{save_name}_it{it}.npz")
     if os.path.exists(path):
         os.remove(path)
-    np.savez(path, lx=lengths[0], ly=lengths[1], lz=lengths[2] if len(lengths) > 2 else 0.0,
-             halo=S.halo, pcur=pcur, possrcs=S.possrcs, posrecs=S.posrecs)
+    np.savez(path, lx=np.float64(lengths[0]), ly=np.float64(lengths[1]),
+             lz=np.float64(lengths[2] if len(lengths) > 2 else 0.0), halo=np.int64(S.halo),
+             pcur=np.asarray(pcur, dtype=np.float64), possrcs=np.asarray(S.possrcs, dtype=np.int64),
+             posrecs=np.asarray(S.posrecs, dtype=np.int64))
+    return path
 
 
 def solve2D(lx, ly, lt, vel, srcs, recs, *, halo=20, rcoef=0.0001, ppw=10.0, freetop=True,

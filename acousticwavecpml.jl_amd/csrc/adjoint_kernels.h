@@ -468,6 +468,24 @@ __global__ __launch_bounds__(256) void adjoint_inject(const AdjIoArgs<T> A, int6
     }
 }
 
+// Fold the per-shot partial gradients of the last reverse steps into the
+// FP64 total: total += sum over shots (in shot order), partials = 0.  The
+// partial sums then run over at most one checkpoint segment in the plan's
+// precision, and the long sum over time and shots is FP64.
+template <typename T>
+__global__ __launch_bounds__(256) void fold_gradient(T* grad, double* total, int64_t field_stride,
+                                                     int nshots) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= field_stride) return;
+    double acc = total[t];
+    for (int s = 0; s < nshots; ++s) {
+        T& g = grad[(int64_t)s * field_stride + t];
+        acc += (double)g;
+        g = T(0);
+    }
+    total[t] = acc;
+}
+
 // Sum of the per-shot gradients into shot 0's accumulator.
 template <typename T>
 __global__ __launch_bounds__(256) void sum_shots(T* grad, int64_t field_stride, int nshots) {

@@ -150,6 +150,10 @@ struct awcpml_plan {
     size_t ck_alloc = 0;
     bool ck_valid = false;
     std::vector<int> ck_rot, ck_par;
+    // reverse steps whose gradient terms are summed in the plan's precision
+    // before they are folded into the FP64 total (0: one checkpoint
+    // segment; -1: never, the whole sum stays in the plan's precision)
+    int64_t grad_fold = 0;
     void* q_cur = nullptr;  // d(pnew)/d(fact) output of the next stencil launch
 
     // two time steps per launch (3D): step t+2's field, psi and xi go to
@@ -164,6 +168,13 @@ struct awcpml_plan {
     bool src_on_boundary = false;    // some source lies on a Dirichlet node
     int src_per_shot_max = 0;
     bool boundary_dirty = false;     // such a source was injected since the reset
+    bool wide = false;               // 64-bit offsets forced (awcpml_set_wide_offsets)
+
+    // configuration generation: bumped whenever something a batch copies
+    // into its argument tables changes (source/receiver buffers and nt,
+    // inverse spacings, active CPML sides, launch shape, spare buffers), so
+    // that a batch built earlier notices and rebuilds its tables
+    uint64_t gen = 0;
 };
 
 namespace {
@@ -340,7 +351,7 @@ int launch_stencil_t(awcpml_plan* P) {
     L.stream = P->stream;
     // 32-bit byte offsets when one shot's field fits in 4 GiB (the slabs
     // are smaller than a field)
-    const bool off64 = (uint64_t)P->field_elems * sizeof(T) >= (1ull << 32);
+    const bool off64 = P->wide || (uint64_t)P->field_elems * sizeof(T) >= (1ull << 32);
     if (P->ndim == 3) awcpml::launch_plan_stencil<T, 3>(A, L, P->prefetch, off64);
     else awcpml::launch_plan_stencil<T, 2>(A, L, P->prefetch, off64);
     HIP_TRY(hipGetLastError());
@@ -401,6 +412,7 @@ int sync_io(awcpml_plan* P) {
         rstart.push_back((int)rlin.size());
     }
     drop_graphs(P->graphs);  // they reference the buffers below
+    ++P->gen;                // and so do the argument tables of batches
     dfree(P->d_src_lin);
     dfree(P->d_src_wav);
     dfree(P->d_src_start);
@@ -472,8 +484,8 @@ int do_step(awcpml_plan* P, int64_t it0) {
 // also were requested for the plan.
 bool two_applies(const awcpml_plan* P) {
     return P->p_spare && !P->q_cur && !P->src_on_boundary &&
-           P->src_per_shot_max <= awcpml::kMaxSrc2 &&
-           !P->boundary_dirty && (uint64_t)P->field_elems * P->esize < (1ull << 32);
+           P->src_per_shot_max <= awcpml::kMaxSrc2 && !P->boundary_dirty && !P->wide &&
+           (uint64_t)P->field_elems * P->esize < (1ull << 32);
 }
 
 bool two_ok(const awcpml_plan* P) { return P->time_block == 2 && two_applies(P); }
@@ -499,6 +511,33 @@ void two_tiling(const awcpml_plan* P, int* rows, int* march, int* prefetch) {
     *prefetch = P->tb_prefetch > 0 ? P->tb_prefetch : 2;
 }
 
+// March chunks of a two-step launch (chunk_start[0..count]), or 0 for
+// uniform chunks of ml planes.  A chunk reads one plane beyond each end
+// (step t+1 there), so the CPML layers 1..halo of each absorbing side of
+// the march axis, with the plane next to them, form chunks of their own:
+// [1, halo+2) and [n-halo-2, n-1).  The interior chunks between then take
+// the march without slab terms.  Each part is cut into near-equal chunks of
+// at most ml planes.
+int two_chunk_list(const awcpml_plan* P, int ml, int* starts) {
+    const int a = P->ndim - 1;
+    const int64_t n = P->n[a], h = P->halo;
+    if (h == 0 || (!P->active[a][0] && !P->active[a][1])) return 0;
+    const int64_t lo = P->active[a][0] ? h + 2 : 1;
+    const int64_t hi = P->active[a][1] ? n - h - 2 : n - 1;
+    if (lo >= hi) return 0;  // the layers fill the axis: uniform chunks
+    const int64_t bounds[4] = {1, lo, hi, n - 1};
+    int count = 0;
+    starts[0] = 1;
+    for (int part = 0; part < 3; ++part) {
+        const int64_t b0 = bounds[part], b1 = bounds[part + 1];
+        if (b1 <= b0) continue;
+        const int64_t pieces = (b1 - b0 + ml - 1) / ml;
+        if (count + pieces > awcpml::kMaxChunks2) return 0;
+        for (int64_t q = 1; q <= pieces; ++q) starts[++count] = (int)(b0 + (b1 - b0) * q / pieces);
+    }
+    return count;
+}
+
 // Arguments of a two-step launch from the current layout: step t+1 reads
 // p[rot] (pold), p[rot+1] (pcur), psi[par] and xi and writes p[rot+2];
 // step t+2 writes the spare field, psi and xi buffers.
@@ -518,13 +557,14 @@ awcpml::Stencil2Args<T> make_args2(const awcpml_plan* P, int64_t it0, int W, int
     A.nsgroups = P->nshots;
     A.tiles_x = (int)((P->n[0] - 2 + awcpml::kOwnX - 1) / awcpml::kOwnX);
     A.march_len = ml;
+    B.nchunk_list = two_chunk_list(P, ml, B.chunk_start);
     if (P->ndim == 3) {
         A.tiles_y = (int)((P->n[1] - 2 + W - 3) / (W - 2));
-        A.nchunks = (int)((P->n[2] - 2 + ml - 1) / ml);
+        A.nchunks = B.nchunk_list > 0 ? B.nchunk_list : (int)((P->n[2] - 2 + ml - 1) / ml);
         A.logical_blocks = A.tiles_x * A.tiles_y * A.nchunks * P->nshots;
     } else {
         // a block's W waves march W consecutive chunks of rows
-        A.nchunks = (int)((P->n[1] - 2 + ml - 1) / ml);
+        A.nchunks = B.nchunk_list > 0 ? B.nchunk_list : (int)((P->n[1] - 2 + ml - 1) / ml);
         A.tiles_y = (A.nchunks + W - 1) / W;
         A.logical_blocks = A.tiles_x * A.tiles_y * P->nshots;
     }
@@ -790,6 +830,7 @@ struct awcpml_batch {
     int64_t it = 0, nt_max = 0, nt_min = 0;
     bool use_graph = false;
     std::vector<GraphEntry> graphs;
+    std::vector<uint64_t> gens;      // each plan's configuration generation the tables hold
 };
 
 namespace {
@@ -895,7 +936,8 @@ int batch_roles(awcpml_batch* B, const Roles& R, int* idx) {
     return AWCPML_OK;
 }
 
-// Drop every table and graph and rebuild the tables of the current roles.
+// Drop every table and graph and rebuild the tables of the current roles
+// (the home roles when there are none).
 int batch_rebuild(awcpml_batch* B) {
     HIP_TRY(hipStreamSynchronize(B->stream));
     drop_graphs(B->graphs);
@@ -903,7 +945,50 @@ int batch_rebuild(awcpml_batch* B) {
     for (auto& t : B->roles) free_tables(t);
     B->roles.clear();
     B->cur = 0;
+    B->gens.clear();
+    for (auto* P : B->plans) B->gens.push_back(P->gen);
     return batch_roles(B, R, &B->cur);
+}
+
+// Some plan was reconfigured since the tables were built: its source and
+// receiver buffers, nt, spacings, active sides, launch shape or spare
+// buffers may differ from what the tables hold.
+bool batch_stale(const awcpml_batch* B) {
+    if (B->gens.size() != B->plans.size()) return true;
+    for (size_t q = 0; q < B->plans.size(); ++q)
+        if (B->plans[q]->io_dirty || B->plans[q]->gen != B->gens[q]) return true;
+    return false;
+}
+
+int batch_stale_error() {
+    return fail(AWCPML_ERR_STATE,
+                "a plan of the batch was reconfigured (sources, receivers, model, CPML, launch "
+                "shape) since the batch was built or reset: call awcpml_batch_reset");
+}
+
+// The batch's values derived from its plans' configuration.  The plans must
+// still share block_y and prefetch.
+int batch_scan(awcpml_batch* B) {
+    const awcpml_plan* P0 = B->plans[0];
+    for (size_t q = 0; q < B->plans.size(); ++q)
+        if (B->plans[q]->block_y != P0->block_y || B->plans[q]->prefetch != P0->prefetch)
+            return fail(AWCPML_ERR_ARG, "plans of a batch must share block_y and prefetch (plan %d differs)",
+                        (int)q);
+    B->block_y = P0->block_y;
+    B->prefetch = P0->prefetch;
+    B->halo_max = 0;
+    B->nt_max = 0;
+    B->nt_min = P0->nt;
+    B->use32 = true;
+    B->has_io = false;
+    for (auto* P : B->plans) {
+        B->halo_max = std::max(B->halo_max, P->halo);
+        B->nt_max = std::max(B->nt_max, P->nt);
+        B->nt_min = std::min(B->nt_min, P->nt);
+        B->use32 = B->use32 && !P->wide && (uint64_t)P->field_elems * P->esize < (1ull << 32);
+        B->has_io = B->has_io || P->has_io;
+    }
+    return AWCPML_OK;
 }
 
 int batch_build(awcpml_batch* B) {
@@ -1124,10 +1209,12 @@ int awcpml_plan_create(int ndim, int64_t nx, int64_t ny, int64_t nz, int halo, i
     int64_t n[3] = {nx, ny, nz};
     for (int a = 0; a < ndim; ++a) {
         if (n[a] < 3) return fail(AWCPML_ERR_ARG, "axis %d has %lld points (need >= 3)", a, (long long)n[a]);
-        if (n[a] < 2 * (int64_t)halo + 2)
-            return fail(AWCPML_ERR_ARG,
-                        "axis %d has %lld points: the two CPML layers of halo %d overlap (need >= %d)", a,
-                        (long long)n[a], halo, 2 * halo + 2);
+        // the right psi layers start at node n-halo-2 (0-based) and must lie
+        // in the grid, as in update_psi_*! (acoustic_2D_xPU.jl:7-31); below
+        // 2*halo+2 points the two slabs overlap and the left side wins
+        if (n[a] < (int64_t)halo + 2)
+            return fail(AWCPML_ERR_ARG, "axis %d has %lld points: halo %d needs >= %d", a,
+                        (long long)n[a], halo, halo + 2);
     }
     awcpml_plan* P = new (std::nothrow) awcpml_plan();
     if (!P) return fail(AWCPML_ERR_NOMEM, "host allocation failed");
@@ -1237,6 +1324,7 @@ int awcpml_set_model(awcpml_plan* P, const double* fact, const double* inv_d, co
     }
     HIP_TRY(hipStreamSynchronize(P->stream));
     P->model_set = true;
+    ++P->gen;
     return AWCPML_OK;
 }
 
@@ -1264,6 +1352,7 @@ int awcpml_set_cpml(awcpml_plan* P, int axis, const double* a_l, const double* a
     P->cpml_set[axis] = true;
     P->ck_valid = false;
     drop_graphs(P->graphs);
+    ++P->gen;
     return AWCPML_OK;
 }
 
@@ -1500,6 +1589,7 @@ int awcpml_set_tiling(awcpml_plan* P, int block_y, int march_len, int prefetch) 
     P->march_len = march_len;
     P->prefetch = prefetch ? prefetch : 2;
     drop_graphs(P->graphs);
+    ++P->gen;
     int st2 = default_tiling(P);
     if (P->block_y % P->shot_group) P->shot_group = 1;
     return st2;
@@ -1515,6 +1605,7 @@ int awcpml_set_shot_group(awcpml_plan* P, int shot_group) {
         return fail(AWCPML_ERR_ARG, "shot_group %d does not divide block_y %d", shot_group, P->block_y);
     P->shot_group = shot_group;
     drop_graphs(P->graphs);
+    ++P->gen;
     return AWCPML_OK;
 }
 
@@ -1563,6 +1654,7 @@ int awcpml_set_time_block(awcpml_plan* P, int steps, int block_rows, int march_l
     P->tb_march = march_len;
     P->tb_prefetch = prefetch;
     drop_graphs(P->graphs);
+    ++P->gen;
     return AWCPML_OK;
 }
 
@@ -1624,23 +1716,18 @@ int awcpml_batch_create(awcpml_plan* const* plans, int count, awcpml_batch** out
             return fail(AWCPML_ERR_STATE,
                         "plan %d's buffers traded roles in two-step launches (alone or in a batch) "
                         "since its last reset: reset it before batching", q);
-        if (plans[q]->block_y != plans[0]->block_y || plans[q]->prefetch != plans[0]->prefetch)
-            return fail(AWCPML_ERR_ARG, "plans of a batch must share block_y and prefetch (plan %d differs)", q);
     }
     awcpml_batch* B = new (std::nothrow) awcpml_batch();
     if (!B) return fail(AWCPML_ERR_NOMEM, "host allocation failed");
     B->plans.assign(plans, plans + count);
     B->ndim = plans[0]->ndim;
     B->dtype = plans[0]->dtype;
-    B->block_y = plans[0]->block_y;
-    B->prefetch = plans[0]->prefetch;
-    B->nt_min = plans[0]->nt;
-    for (auto* P : B->plans) {
-        B->halo_max = std::max(B->halo_max, P->halo);
-        B->nt_max = std::max(B->nt_max, P->nt);
-        B->nt_min = std::min(B->nt_min, P->nt);
-        B->use32 = B->use32 && (uint64_t)P->field_elems * P->esize < (1ull << 32);
-        B->has_io = B->has_io || P->has_io;
+    {
+        int st = batch_scan(B);
+        if (st) {
+            delete B;
+            return st;
+        }
     }
     // two-step launches when every plan asks for them with one launch shape
     bool two = true;
@@ -1684,8 +1771,21 @@ int awcpml_batch_reset(awcpml_batch* B) {
     for (auto* P : B->plans) {
         int st = awcpml_reset(P);
         if (st) return st;
+        if ((st = ready(P))) return st;
     }
     B->it = 0;
+    if (batch_stale(B)) {
+        // a plan was reconfigured: rebuild every table from the plans' new
+        // buffers and values (the plans are at home after their reset)
+        HIP_TRY(hipStreamSynchronize(B->stream));
+        drop_graphs(B->graphs);
+        for (auto& t : B->roles) free_tables(t);
+        B->roles.clear();
+        B->cur = 0;
+        int st = batch_scan(B);
+        if (st) return st;
+        return batch_build(B);
+    }
     return batch_roles(B, home_roles(), &B->cur);
 }
 
@@ -1693,8 +1793,9 @@ int awcpml_batch_advance(awcpml_batch* B, int64_t nsteps) {
     g_err.clear();
     if (!B) return fail(AWCPML_ERR_ARG, "batch is NULL");
     if (nsteps < 0) return fail(AWCPML_ERR_ARG, "nsteps < 0");
+    if (batch_stale(B)) return batch_stale_error();
     for (auto* P : B->plans)
-        if (P->it != std::min(B->it, P->nt) || P->io_dirty ||
+        if (P->it != std::min(B->it, P->nt) ||
             !(get_layout(P) == roles_layout(P, B->roles[B->cur].r, P->it)))
             return fail(AWCPML_ERR_STATE, "a plan of the batch was stepped or changed on its own; "
                                           "rebuild the batch or call awcpml_batch_reset");
@@ -1721,6 +1822,7 @@ int awcpml_batch_set_graph(awcpml_batch* B, int enable) {
 int awcpml_batch_time_advance(awcpml_batch* B, int64_t nsteps, double* ms_total) {
     g_err.clear();
     if (!B || !ms_total || nsteps < 0) return fail(AWCPML_ERR_ARG, "bad argument");
+    if (batch_stale(B)) return batch_stale_error();
     if (B->it + nsteps > B->nt_max) return fail(AWCPML_ERR_STATE, "advance past the longest run");
     int st;
     if (nsteps > 0 && B->use_graph) {  // capture outside the timed region
@@ -1748,6 +1850,7 @@ int awcpml_batch_time_advance(awcpml_batch* B, int64_t nsteps, double* ms_total)
 int awcpml_batch_time_stencil(awcpml_batch* B, int nlaunches, double* ms_total) {
     g_err.clear();
     if (!B || !ms_total || nlaunches < 1) return fail(AWCPML_ERR_ARG, "bad argument");
+    if (batch_stale(B)) return batch_stale_error();
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -1795,6 +1898,7 @@ int awcpml_batch_get_time_block(const awcpml_batch* B, int* steps, int* block_ro
 int awcpml_batch_time_stencil2(awcpml_batch* B, int nlaunches, double* ms_total) {
     g_err.clear();
     if (!B || !ms_total || nlaunches < 1) return fail(AWCPML_ERR_ARG, "bad argument");
+    if (batch_stale(B)) return batch_stale_error();
     if (!batch_two_ok(B)) return fail(AWCPML_ERR_STATE, "two-step launches do not apply to this batch");
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
@@ -1812,6 +1916,16 @@ int awcpml_batch_time_stencil2(awcpml_batch* B, int nlaunches, double* ms_total)
     (void)hipEventDestroy(e1);
     *ms_total = ms;
     return st;
+}
+
+int awcpml_set_wide_offsets(awcpml_plan* P, int enable) {
+    g_err.clear();
+    int st = check_plan(P);
+    if (st) return st;
+    P->wide = enable != 0;
+    drop_graphs(P->graphs);
+    ++P->gen;
+    return AWCPML_OK;
 }
 
 int awcpml_get_tiling(const awcpml_plan* P, int* block_y, int* march_len, int* prefetch) {
@@ -1901,6 +2015,7 @@ struct AdjBuffers {
     void* be[3][2][2] = {};
     void* q = nullptr;
     void* grad = nullptr;
+    void* grad64 = nullptr;
     void* tbar = nullptr;
     void* node_lin = nullptr;
     void* node_start = nullptr;
@@ -1916,6 +2031,7 @@ struct AdjBuffers {
                 }
         dfree(q);
         dfree(grad);
+        dfree(grad64);
         dfree(tbar);
         dfree(node_lin);
         dfree(node_start);
@@ -1998,6 +2114,19 @@ int adjoint_t(awcpml_plan* P, const double* tbar, double* fbar) {
     if ((st = dalloc(&Bf.q, fb * (size_t)K))) return st;
     if ((st = dalloc(&Bf.grad, fb))) return st;
     HIP_TRY(hipMemsetAsync(Bf.grad, 0, fb, P->stream));
+    const bool fold = P->grad_fold >= 0;
+    const int64_t fold_every = P->grad_fold > 0 ? P->grad_fold : K;
+    const int64_t nb_fold = (P->field_elems + 255) / 256;
+    if (fold) {
+        if ((st = dalloc(&Bf.grad64, (size_t)P->field_elems * 8))) return st;
+        HIP_TRY(hipMemsetAsync(Bf.grad64, 0, (size_t)P->field_elems * 8, P->stream));
+    }
+    auto fold_now = [&]() -> int {
+        awcpml::fold_gradient<T><<<dim3((unsigned)nb_fold), dim3(256), 0, P->stream>>>(
+            (T*)Bf.grad, (double*)Bf.grad64, P->field_elems, P->nshots);
+        HIP_TRY(hipGetLastError());
+        return AWCPML_OK;
+    };
 
     // trace cotangents and the distinct interior receiver nodes per shot
     const size_t ntr = (size_t)P->n_rec_total * nt;
@@ -2092,7 +2221,17 @@ int adjoint_t(awcpml_plan* P, const double* tbar, double* fbar) {
             }
             lr = (lr + 2) % 3;
             par ^= 1;
+            if (fold && (s == c0 + 1 || (c1 - s + 1) % fold_every == 0) && (st = fold_now())) return st;
         }
+    }
+    const int64_t nx = P->n[0], rows = P->n[1] * P->n[2];
+    if (fold) {
+        std::vector<double> host((size_t)P->field_elems);
+        HIP_TRY(hipMemcpyAsync(host.data(), Bf.grad64, host.size() * 8, hipMemcpyDeviceToHost, P->stream));
+        HIP_TRY(hipStreamSynchronize(P->stream));
+        for (int64_t r = 0; r < rows; ++r)
+            for (int64_t i = 0; i < nx; ++i) fbar[r * nx + i] = host[(size_t)(r * P->px + i)];
+        return AWCPML_OK;
     }
     if (P->nshots > 1) {
         const int64_t nb = (P->field_elems + 255) / 256;
@@ -2103,7 +2242,6 @@ int adjoint_t(awcpml_plan* P, const double* tbar, double* fbar) {
     std::vector<T> host((size_t)P->field_elems);
     HIP_TRY(hipMemcpyAsync(host.data(), Bf.grad, host.size() * sizeof(T), hipMemcpyDeviceToHost, P->stream));
     HIP_TRY(hipStreamSynchronize(P->stream));
-    const int64_t nx = P->n[0], rows = P->n[1] * P->n[2];
     for (int64_t r = 0; r < rows; ++r)
         for (int64_t i = 0; i < nx; ++i) fbar[r * nx + i] = (double)host[(size_t)(r * P->px + i)];
     return AWCPML_OK;
@@ -2148,6 +2286,15 @@ int awcpml_checkpoint_run(awcpml_plan* P, int64_t every) {
     }
     HIP_TRY(hipStreamSynchronize(P->stream));
     P->ck_valid = true;
+    return AWCPML_OK;
+}
+
+int awcpml_set_gradient_fold(awcpml_plan* P, int64_t steps) {
+    g_err.clear();
+    int st = check_plan(P);
+    if (st) return st;
+    if (steps < -1) return fail(AWCPML_ERR_ARG, "steps must be >= -1 (got %lld)", (long long)steps);
+    P->grad_fold = steps;
     return AWCPML_OK;
 }
 

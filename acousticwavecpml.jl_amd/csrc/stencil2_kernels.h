@@ -38,6 +38,7 @@ namespace awcpml {
 
 constexpr int kOwnX = kTileX - 2;  // owned x nodes per wave
 constexpr int kMaxSrc2 = 32;       // sources per shot a two-step launch takes
+constexpr int kMaxChunks2 = 32;    // march chunks a two-step launch lists
 
 template <typename T>
 struct Stencil2Args {
@@ -52,7 +53,26 @@ struct Stencil2Args {
     const int* src_start;    // nshots+1 CSR offsets
     int64_t nt;
     int64_t it0;             // 0-based index of step t+1
+    // march chunks: chunk c owns planes (2D: rows) [chunk_start[c],
+    // chunk_start[c+1]); 0 chunks listed: chunk c starts at 1 + c*march_len.
+    // The host cuts the CPML layers of the march axis into chunks of their
+    // own, so the interior chunks run the march without the slab terms.
+    int nchunk_list;
+    int chunk_start[kMaxChunks2 + 1];
 };
+
+// The planes (2D: rows) chunk c of a launch owns: [*kbeg, *kend).
+template <typename T>
+__device__ __forceinline__ void two_chunk(const Stencil2Args<T>& B, int chunk, int64_t n_march,
+                                          int* kbeg, int* kend) {
+    if (B.nchunk_list > 0) {
+        *kbeg = B.chunk_start[chunk];
+        *kend = B.chunk_start[chunk + 1];
+    } else {
+        *kbeg = 1 + chunk * B.s.march_len;
+        *kend = (int)min((int64_t)*kbeg + B.s.march_len, n_march - 1);
+    }
+}
 
 // Lane shifts across the 64-lane wavefront with DPP (VALU moves, no LDS
 // traffic): from_below gives lane i the value of lane i-1, from_above the
@@ -192,7 +212,7 @@ __device__ __forceinline__ void two_second(const March<T, Off, 3>& M, const TwoO
         damp += f * e.t;
         if (own) {
             stg(M.sa.p.psi_wr, M.sa.o_psi + (Off)k * M.sa.st_psi, e.hi);
-            if (M.sa.m == 1) stg(M.sa.p.psi_wr, M.sa.o_psi1 + (Off)k * M.sa.st_psi, e.lo);
+            if (M.sa.first) stg(M.sa.p.psi_wr, M.sa.o_psi1 + (Off)k * M.sa.st_psi, e.lo);
             stg(O.xi_x, M.sa.o_xi + (Off)k * M.sa.st_xi, e.xn);
         }
     }
@@ -201,7 +221,7 @@ __device__ __forceinline__ void two_second(const March<T, Off, 3>& M, const TwoO
         damp += f * e.t;
         if (own) {
             stg(M.sb.p.psi_wr, M.sb.o_psi + (Off)k * M.sb.st_psi, e.hi);
-            if (M.sb.m == 1) stg(M.sb.p.psi_wr, M.sb.o_psi1 + (Off)k * M.sb.st_psi, e.lo);
+            if (M.sb.first) stg(M.sb.p.psi_wr, M.sb.o_psi1 + (Off)k * M.sb.st_psi, e.lo);
             stg(O.xi_y, M.sb.o_xi + (Off)k * M.sb.st_xi, e.xn);
         }
     }
@@ -212,7 +232,7 @@ __device__ __forceinline__ void two_second(const March<T, Off, 3>& M, const TwoO
         if (own) {
             T* pw = M.pm_side[ZS < 0 ? 0 : ZS].psi_wr;
             stg(pw, M.col + (Off)mz * M.sm, e.hi);
-            if (mz == 1) stg(pw, M.col + (Off)(mz - 1) * M.sm, e.lo);
+            if (mz == M.mfirst[ZS < 0 ? 0 : ZS]) stg(pw, M.col + (Off)(mz - 1) * M.sm, e.lo);
             stg(O.xi_z[ZS < 0 ? 0 : ZS], M.col + (Off)(mz - 1) * M.sm, e.xn);
         }
     }
@@ -505,8 +525,8 @@ __device__ __forceinline__ void stencil3d_two_block(const Stencil2Args<T>& B, in
     const int64_t jc = j < 1 ? 1 : (j > ny - 2 ? ny - 2 : j);
     const bool in1 = i == ic && j == jc;
     const bool own = in1 && lane >= 1 && lane <= kOwnX && wy >= 1 && wy <= W - 2;
-    const int kbeg = 1 + chunk * A.march_len;
-    const int kend = (int)min((int64_t)kbeg + A.march_len, nz - 1);
+    int kbeg, kend;
+    two_chunk(B, chunk, nz, &kbeg, &kend);
 
     constexpr Off E = (Off)sizeof(T);
     const int64_t fo = (int64_t)shot * A.field_stride;
@@ -538,6 +558,8 @@ __device__ __forceinline__ void stencil3d_two_block(const Stencil2Args<T>& B, in
     for (int s = 0; s < 2; ++s) M.pm_side[s] = slab_ptrs(A, 2, s, shot);
     M.mshift[0] = 0;
     M.mshift[1] = (int)(nz - halo - 2);
+    M.mfirst[0] = 1;
+    M.mfirst[1] = right_first(nz, halo);
 
     TwoOut<T, Off> O;
     O.p1 = B.p1 + fo;
@@ -655,7 +677,7 @@ __device__ __forceinline__ void two2_second(const March<T, Off, 2>& M, const Two
         damp += f * e.t;
         if (own) {
             stg(M.sa.p.psi_wr, M.sa.o_psi + (Off)k * M.sa.st_psi, e.hi);
-            if (M.sa.m == 1) stg(M.sa.p.psi_wr, M.sa.o_psi1 + (Off)k * M.sa.st_psi, e.lo);
+            if (M.sa.first) stg(M.sa.p.psi_wr, M.sa.o_psi1 + (Off)k * M.sa.st_psi, e.lo);
             stg(O.xi_x, M.sa.o_xi + (Off)k * M.sa.st_xi, e.xn);
         }
     }
@@ -666,7 +688,7 @@ __device__ __forceinline__ void two2_second(const March<T, Off, 2>& M, const Two
         if (own) {
             T* pw = M.pm_side[ZS < 0 ? 0 : ZS].psi_wr;
             stg(pw, M.col + (Off)mz * M.sm, e.hi);
-            if (mz == 1) stg(pw, M.col + (Off)(mz - 1) * M.sm, e.lo);
+            if (mz == M.mfirst[ZS < 0 ? 0 : ZS]) stg(pw, M.col + (Off)(mz - 1) * M.sm, e.lo);
             stg(O.xi_z[ZS < 0 ? 0 : ZS], M.col + (Off)(mz - 1) * M.sm, e.xn);
         }
     }
@@ -844,8 +866,8 @@ __device__ __forceinline__ void stencil2d_two_block(const Stencil2Args<T>& B, in
     const int64_t ic = i < 1 ? 1 : (i > nx - 2 ? nx - 2 : i);
     const bool in1 = i == ic;
     const bool own = in1 && lane >= 1 && lane <= kOwnX;
-    const int kbeg = 1 + chunk * A.march_len;
-    const int kend = (int)min((int64_t)kbeg + A.march_len, ny - 1);
+    int kbeg, kend;
+    two_chunk(B, chunk, ny, &kbeg, &kend);
 
     constexpr Off E = (Off)sizeof(T);
     const int64_t fo = (int64_t)shot * A.field_stride;
@@ -876,6 +898,8 @@ __device__ __forceinline__ void stencil2d_two_block(const Stencil2Args<T>& B, in
     for (int s = 0; s < 2; ++s) M.pm_side[s] = slab_ptrs(A, 1, s, shot);
     M.mshift[0] = 0;
     M.mshift[1] = (int)(ny - halo - 2);
+    M.mfirst[0] = 1;
+    M.mfirst[1] = right_first(ny, halo);
 
     TwoOut<T, Off> O;
     O.p1 = B.p1 + fo;

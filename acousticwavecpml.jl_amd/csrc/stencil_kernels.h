@@ -28,9 +28,10 @@
 //  * the psi recursions are fused into the pressure update.  psi is
 //    double-buffered (psi_rd holds step n-1, psi_wr receives step n): the
 //    thread at node m recomputes psi[m] and psi[m-1] from psi_rd and the
-//    three pressures it holds, then writes psi[m] (and psi[0] when m == 1),
-//    so no second launch and no grid-wide ordering are needed and each psi
-//    value is produced with exactly the reference's expression;
+//    three pressures it holds, then writes psi[m] (and psi[m-1] at the
+//    side's first layer, psi[0] unless the two sides' slabs overlap), so no
+//    second launch and no grid-wide ordering are needed and each psi value
+//    is produced with exactly the reference's expression;
 //  * only threads inside a CPML slab touch psi/xi, so their traffic is paid
 //    in the slabs only; a side whose coefficients are a = 0, b_K = 1 (the
 //    free top) is skipped entirely;
@@ -135,7 +136,10 @@ __device__ __forceinline__ void stg_stream(T* base, Off byte_off, T v) {
 // Mirrors `if i <= halo+1 ... elseif i >= nx-halo` (1-based) with the
 // 0-based node index q: left if q <= halo, right if q >= n-halo-1, where
 // the right layer is m = q - (n-halo-2).  An inactive side still takes its
-// branch (so the other side is not tested), as in the reference.
+// branch (so the other side is not tested), as in the reference.  On a
+// grid with n < 2*halo+2 the two slabs overlap and the left side wins
+// (acoustic_2D_xPU.jl:45-56): the right side's first node is then q =
+// halo+1, at layer right_first(n, halo) > 1.
 __device__ __forceinline__ int classify(int64_t q, int64_t n, int64_t halo,
                                         const int* active, int* m) {
     if (q <= halo) {
@@ -150,6 +154,11 @@ __device__ __forceinline__ int classify(int64_t q, int64_t n, int64_t halo,
 
 // The six CPML coefficients one layer m needs: b_K_h[m], a_h[m],
 // b_K_h[m-1], a_h[m-1] (staggered) and b_K[m-1], a[m-1] (nodal).
+__device__ __forceinline__ int right_first(int64_t n, int64_t halo) {
+    const int64_t m = 2 * halo + 3 - n;
+    return m > 1 ? (int)m : 1;
+}
+
 template <typename T>
 struct LayerCoef {
     T bKh, ah, bKh1, ah1, bK1, a1;
@@ -171,20 +180,21 @@ __device__ __forceinline__ LayerCoef<T> layer_coef(const CpmlSide<T>& s, int m) 
 // values psi_old = psi[m], psi_old1 = psi[m-1] and xi_old = xi[m-1]
 // (already loaded).  pm, pc, pp: pressure before, at and after the node
 // along the axis; d2: the node's second derivative along the axis; f: fact.
-// Stores the new psi[m] (and psi[m-1] when m == 1) and xi[m-1]; returns
+// Stores the new psi[m] (and psi[m-1] when `first`: the node is its side's
+// first slab node, whose psi[m-1] no other node writes) and xi[m-1]; returns
 // dpsi + xi, which the caller scales by fact into the reference's damping
 // term damp += fact*(dpsi + xi) (acoustic_2D_xPU.jl:7-18 and 45-56 for
 // x-left; the other sides are identical up to index shifts).
 template <typename T, typename Off>
 __device__ __forceinline__ T cpml_apply(const LayerCoef<T>& c, T* psi_wr, T* xi_arr, Off psi_m,
-                                        Off psi_m1, Off xi_m1, int m, T psi_old,
+                                        Off psi_m1, Off xi_m1, bool first, T psi_old,
                                         T psi_old1, T xi_old, T pm, T pc, T pp, T d2,
                                         T inv_d, T f, bool st = true) {
     // update_psi_*!: psi[m] = b_K_h[m]*psi[m] + a_h[m]*(p[m+1]-p[m])*_d
     const T psi_here = c.bKh * psi_old + c.ah * (pp - pc) * inv_d;
     const T psi_prev = c.bKh1 * psi_old1 + c.ah1 * (pc - pm) * inv_d;
     if (st) stg(psi_wr, psi_m, psi_here);
-    if (st && m == 1) stg(psi_wr, psi_m1, psi_prev);
+    if (st && first) stg(psi_wr, psi_m1, psi_prev);
     // update_p!: dpsi, in-place xi, damping
     const T dpsi = (psi_here - psi_prev) * inv_d;
     const T xin = c.bK1 * xi_old + c.a1 * (d2 + dpsi);
@@ -219,6 +229,7 @@ template <typename T, typename Off>
 struct ColSlab {
     bool in;            // this lane's column is in the slab (1 <= m <= halo)
     bool own;           // lane-shuffle x path: this lane computes psi[m], 0 <= m <= halo
+    bool first;         // in, and the first slab node of its side (also writes psi[m-1])
     int m;
     SlabPtrs<T> p;
     LayerCoef<T> c;
@@ -285,6 +296,7 @@ struct March {
     ColSlab<T, Off> sa, sb;   // x slab, y slab (3D)
     SlabPtrs<T> pm_side[2];   // march-axis slab pointers per side
     int mshift[2];            // march layer = node - mshift[side]
+    int mfirst[2];            // march layer of each side's first slab node
     const T* lds;             // march-axis coefficients
     int h1;
     T* qout;                  // optional d(pnew)/d(fact) output (this shot)
@@ -364,7 +376,7 @@ __device__ __forceinline__ void compute_node(const March<T, Off, NDIM>& M, int k
         if (M.sa.in) {
             const T t = cpml_apply(M.sa.c, M.sa.p.psi_wr, M.sa.p.xi, M.sa.o_psi + (Off)k * M.sa.st_psi,
                                    M.sa.o_psi1 + (Off)k * M.sa.st_psi, M.sa.o_xi + (Off)k * M.sa.st_xi,
-                                   M.sa.m, q.psa0, q.psa1, q.xia, q.xm, pc, q.xp, d2a, M.ia, f);
+                                   M.sa.first, q.psa0, q.psa1, q.xia, q.xm, pc, q.xp, d2a, M.ia, f);
             damp += f * t;
             slab += t;
         }
@@ -372,7 +384,7 @@ __device__ __forceinline__ void compute_node(const March<T, Off, NDIM>& M, int k
     if (NDIM == 3 && SB) {
         const T t = cpml_apply(M.sb.c, M.sb.p.psi_wr, M.sb.p.xi, M.sb.o_psi + (Off)k * M.sb.st_psi,
                                M.sb.o_psi1 + (Off)k * M.sb.st_psi, M.sb.o_xi + (Off)k * M.sb.st_xi,
-                               M.sb.m, q.psb0, q.psb1, q.xib, q.ym, pc, q.yp, d2b, M.ib, f, st);
+                               M.sb.first, q.psb0, q.psb1, q.xib, q.ym, pc, q.yp, d2b, M.ib, f, st);
         damp += f * t;
         slab += t;
     }
@@ -381,8 +393,9 @@ __device__ __forceinline__ void compute_node(const March<T, Off, NDIM>& M, int k
         const LayerCoef<T> cz = lds_coef(M.lds, M.h1, mside, mz);
         const SlabPtrs<T>& P = M.pm_side[mside];
         const T t = cpml_apply(cz, P.psi_wr, P.xi, M.col + (Off)mz * M.sm,
-                               M.col + (Off)(mz - 1) * M.sm, M.col + (Off)(mz - 1) * M.sm, mz,
-                               q.psm0, q.psm1, q.xim, pm, pc, pp, d2m, M.imm, f, st);
+                               M.col + (Off)(mz - 1) * M.sm, M.col + (Off)(mz - 1) * M.sm,
+                               mz == M.mfirst[mside], q.psm0, q.psm1, q.xim, pm, pc, pp, d2m, M.imm,
+                               f, st);
         damp += f * t;
         slab += t;
     }
@@ -575,8 +588,11 @@ __device__ __forceinline__ void march_segment_far(const March<T, Off, NDIM>& M, 
 template <typename T, typename Off, int NDIM, int D, int SA, bool SB>
 __device__ __forceinline__ void march_column(const March<T, Off, NDIM>& M, const int* active_m,
                                              int64_t n_m, int64_t halo, int kbeg, int kend) {
+    // the high side starts after the low side's nodes even when the low
+    // side is idle (overlapping slabs: the low branch wins, classify())
     const int lo_end = active_m[0] ? (int)min((int64_t)kend, halo + 1) : kbeg;
-    const int hi_beg = active_m[1] ? (int)max((int64_t)kbeg, n_m - halo - 1) : kend;
+    const int hi_beg =
+        active_m[1] ? (int)max((int64_t)kbeg, max(n_m - halo - 1, halo + 1)) : kend;
     const int a = max(kbeg, lo_end), b = min(kend, max(a, hi_beg));
     if constexpr (D >= 4) {
         march_segment_far<T, Off, NDIM, D, SA, SB, true>(M, kbeg, min(lo_end, kend), 0);
@@ -605,6 +621,7 @@ __device__ __forceinline__ void col_slab(const StencilArgs<T>& A, int axis, int6
         m = 1;
     }
     S.own = false;
+    S.first = S.in && m == (side ? right_first(n, A.halo) : 1);
     S.m = m;
     S.p = slab_ptrs(A, axis, side, shot);
     S.c = LayerCoef<T>{};
@@ -637,6 +654,7 @@ __device__ __forceinline__ void col_slab_xfast(const StencilArgs<T>& A, int64_t 
     }
     S.own = side >= 0;
     S.in = S.own && m >= 1;
+    S.first = false;  // the owning lanes write every psi layer themselves
     if (!S.own) {
         side = 0;
         m = 1;  // a valid dummy location
@@ -730,6 +748,8 @@ __device__ __forceinline__ void stencil3d_block(const StencilArgs<T>& A, int lb,
     for (int s = 0; s < 2; ++s) M.pm_side[s] = slab_ptrs(A, 2, s, shot);
     M.mshift[0] = 0;
     M.mshift[1] = (int)(nz - halo - 2);
+    M.mfirst[0] = 1;
+    M.mfirst[1] = right_first(nz, halo);
 
     const bool xs = A.x_fast ? __any(M.sa.own) : __any(M.sa.in);
     const bool ys = M.sb.in;  // wave-uniform
@@ -800,6 +820,8 @@ __device__ __forceinline__ void stencil2d_block(const StencilArgs<T>& A, int lb,
     for (int s = 0; s < 2; ++s) M.pm_side[s] = slab_ptrs(A, 1, s, shot);
     M.mshift[0] = 0;
     M.mshift[1] = (int)(ny - halo - 2);
+    M.mfirst[0] = 1;
+    M.mfirst[1] = right_first(ny, halo);
 
     if (A.x_fast && __any(M.sa.own))
         march_column<T, Off, 2, D, 1, false>(M, A.active[1], ny, halo, jbeg, jend);

@@ -51,17 +51,18 @@ def algorithmic_bytes(n, halo, freetop, esize, nshots):
 
 
 def build_plan(S, dtype, nshots):
+    """Plan of `nshots` shots on the workload's model: shot 0 at the
+    configured source, the others spread along x at the source depth, like
+    a line survey."""
     import awcpml
     P = awcpml.Plan(S.n, S.halo, dtype=dtype, nshots=nshots)
     P.set_model(S.fact, S.inv_d, S.inv_d2)
     for a in range(S.ndim):
         P.set_cpml(a, S.coef[a])
     for s in range(nshots):
-        # shots spread along x at the source depth, like a line survey
         pos = S.possrcs.copy()
-        pos[0, 0] = 1 + S.halo + 1 + (s * (S.n[0] - 2 * S.halo - 4)) // max(nshots, 1)
-        if nshots == 1:
-            pos = S.possrcs
+        if s > 0:
+            pos[0, 0] = 1 + S.halo + 1 + (s * (S.n[0] - 2 * S.halo - 4)) // nshots
         P.set_sources(s, pos, S.dt2srctf)
         P.set_receivers(s, S.posrecs)
     return P
@@ -82,7 +83,7 @@ def time_variant(S, dtype, nshots, steps, warmup, freetop, kernel_launches=50, t
         P.reset()
         P.advance(S.nt)
     t = (time.perf_counter() - t0) / max(steps, 1)
-    traces0 = P.traces(0)
+    traces0, field0 = P.traces(0), P.field(0, 1)
     # dominant kernel alone, HIP events on the plan's stream: the two-step
     # kernel (two time steps per launch) when the tuner chose it
     tb = P.time_block()
@@ -108,35 +109,111 @@ def time_variant(S, dtype, nshots, steps, warmup, freetop, kernel_launches=50, t
         "tiling": {"block_y": tiling[0], "march_len": tiling[1], "prefetch": tiling[2],
                    "shot_group": tiling[3]},
         "time_block": {"steps": tb[0], "block_rows": tb[1], "march_len": tb[2], "prefetch": tb[3]},
-        "_traces0": traces0,
+        "_traces0": traces0, "_field0": field0,
     }
 
 
-def cpu_baseline(case, max_seconds=20.0):
-    """Oracle (C FP64 restatement of the reference's Threads path) on the
-    same grid for a bounded number of time steps."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(case, gpu, case2d=None, gpu2d=None):
+    """The oracle (C/OpenMP FP64 restatement of the reference's Threads
+    path) on the host cores: the 3D workload in full on all cores, a short
+    sample on one core, and the 2D workload in full.  As the checker, its
+    outputs also give the GPU results' errors: FP64 bit for bit, FP32 as
+    relative L2 of the traces and of the final field."""
     import oracle
-    o = oracle.OracleProblem(case["n"], case["l"], case["lt"], case["vel"], case["srcpos"],
-                             case["t0s"], case["f0"], case["recpos"], halo=case["halo"],
-                             rcoef=case["rcoef"], ppw=case["ppw"], freetop=case["freetop"])
-    N = 1
-    for v in case["n"]:
-        N *= v
-    o.run(3)  # first touch of the arrays
+    from cases import oracle_problem, rel_l2
+    N = int(np.prod(case["n"]))
+    cores = oracle.set_threads(0)
+    o = oracle_problem(case)
     t0 = time.perf_counter()
-    o.run(3)
-    per = (time.perf_counter() - t0) / 3
-    nsteps = max(2, min(o.nt - 6, int(max_seconds / max(per, 1e-6))))
-    t0 = time.perf_counter()
-    done = o.run(nsteps)
+    done = o.run()
     t = time.perf_counter() - t0
+    tr, pend = o.traces(), o.pend()
     o.close()
-    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
-    return {
-        "value": done * N / t / 1e9, "unit": "shot·Gpts/s", "cores": cores, "kind": "port",
-        "sample": f"plain3d {case['n'][0]}^3 FP64 single shot, time steps 7..{6 + done} of "
-                  f"{o.nt} ({t:.1f} s), C/OpenMP oracle",
+    parity = {
+        "fp64_plain_bitwise_traces": bool(np.array_equal(gpu["f64"]["_traces0"], tr)),
+        "fp64_plain_bitwise_field": bool(np.array_equal(gpu["f64"]["_field0"], pend)),
     }
+    for key, v in gpu.items():
+        if key == "f64":
+            continue
+        parity[f"{key}_trace_rel_l2"] = rel_l2(v["_traces0"], tr)
+        parity[f"{key}_field_rel_l2"] = rel_l2(v["_field0"], pend)
+    # one core, a short sample of the same workload
+    oracle.set_threads(1)
+    o = oracle_problem(case)
+    o.run(2)
+    t1 = time.perf_counter()
+    n1 = o.run(8)
+    t_one = time.perf_counter() - t1
+    o.close()
+    oracle.set_threads(0)
+    out = {
+        "value": done * N / t / 1e9, "unit": "shot·Gpts/s", "cores": cores, "kind": "port",
+        "sample": f"plain3d {case['n'][0]}^3 FP64 single shot, all {done} time steps ({t:.1f} s), "
+                  f"C/OpenMP oracle on {cores} threads",
+        "cpu_model": _cpu_model(),
+        "one_core": {"value": n1 * N / t_one / 1e9, "unit": "shot·Gpts/s",
+                     "sample": f"time steps 3..{2 + n1} of the same solve on 1 thread ({t_one:.1f} s)"},
+        "parity_vs_oracle": parity,
+    }
+    if case2d is not None:
+        o = oracle_problem(case2d)
+        N2 = int(np.prod(case2d["n"]))
+        t2 = time.perf_counter()
+        d2 = o.run()
+        t2 = time.perf_counter() - t2
+        out["plain2d"] = {
+            "value": d2 * N2 / t2 / 1e9, "unit": "shot·Gpts/s", "cores": cores,
+            "sample": f"plain2d {case2d['n'][0]}^2 FP64, all {d2} time steps ({t2:.2f} s)",
+            "gpu_fp64_bitwise_traces": bool(gpu2d is not None and
+                                            np.array_equal(gpu2d["_traces0"], o.traces())),
+            "gpu_fp64_bitwise_field": bool(gpu2d is not None and
+                                           np.array_equal(gpu2d["_field0"], o.pend())),
+        }
+        o.close()
+    return out
+
+
+def bench_plain2d(steps=10):
+    """The 2D drop-in workload (298^2, FP64, one shot, nt = 1000): whole
+    solves with the time loop in a HIP graph, autotuned launch shape."""
+    import awcpml
+    from cases import plain2d
+    case = plain2d()
+    S = _setup(case)
+    P = build_plan(S, "f64", 1)
+    tune = awcpml.autotune(P, launches=5)
+    P.set_graph(True)
+    P.reset()
+    P.advance(S.nt)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        P.reset()
+        P.advance(S.nt)
+    t = (time.perf_counter() - t0) / steps
+    tr, fld = P.traces(0), P.field(0, 1)
+    P.reset()
+    ms = P.time_stencil(50) / 50
+    P.close()
+    N = S.n[0] * S.n[1]
+    B, _ = algorithmic_bytes(S.n, S.halo, False, 8, 1)
+    return case, {"workload": f"plain2d: 2D {S.n[0]}^2 (256^2 interior + 20 halo), FP64, one shot, "
+                              f"nt={S.nt}", "s_per_solve": round(t, 5),
+                  "shot_gpts_per_s": round(N * S.nt / t / 1e9, 3),
+                  "kernel_ms_per_step": round(ms, 5), "kernel_gbs": round(B / ms / 1e6, 1),
+                  "tiling": [tune["block_y"], tune["march_len"], tune["prefetch"]],
+                  "time_block": tune["time_block"], "_traces0": tr, "_field0": fld}
 
 
 def _setup(case, vel=None, srcpos=None):
@@ -401,7 +478,6 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched-2D, uneven-3D and FWI-gradient measurements")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--profile-kernel", action="store_true",
                     help="profiler target: stencil launches of the FP32 S-shot variant only")
     ap.add_argument("--profile-dtype", default="f32", choices=["f32", "f64"])
@@ -467,7 +543,9 @@ def main():
     vb = time_variant(S, "f32", args.shots, args.steps, args.warmup, case["freetop"])
     print(f"[bench] fp32 {args.shots} shots: {vb['shot_gpts_per_s']:.2f} shot*Gpts/s", file=sys.stderr,
           flush=True)
-    fp32_err = rel_l2(v32["_traces0"], v64["_traces0"])
+    print("[bench] plain2d (2D drop-in workload)", file=sys.stderr, flush=True)
+    case2d, v2d = bench_plain2d()
+    print(f"[bench] plain2d fp64: {v2d['shot_gpts_per_s']:.2f} shot*Gpts/s", file=sys.stderr, flush=True)
 
     main_v = vb
     kname = ("stencil3d_two<float>" if main_v["steps_per_launch"] == 2 else "stencil3d<float>")
@@ -500,7 +578,7 @@ def main():
             "tiling": main_v["tiling"],
             "time_block": main_v["time_block"],
             "fp32_vs_fp64_plain": round(main_v["shot_gpts_per_s"] / v64["shot_gpts_per_s"], 3),
-            "fp32_trace_rel_l2_vs_fp64": fp32_err,
+            "plain2d": {k: v for k, v in v2d.items() if not k.startswith("_")},
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -521,7 +599,13 @@ def main():
         out["config"]["extras"] = extras
     if not args.no_cpu:
         print("[bench] cpu baseline (oracle)", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(
+            case, {"f64": v64, "fp32_single": v32, f"fp32_{args.shots}shot_shot0": vb},
+            case2d, v2d)
+        # the FP32 results' errors against the FP64 oracle
+        par = out["cpu_baseline"]["parity_vs_oracle"]
+        out["config"]["fp32_trace_rel_l2_vs_oracle"] = par["fp32_single_trace_rel_l2"]
+        out["config"]["fp32_field_rel_l2_vs_oracle"] = par["fp32_single_field_rel_l2"]
     print(json.dumps(out))
 
 

@@ -49,8 +49,10 @@ extern "C" {
 typedef struct awcpml_plan awcpml_plan;
 
 /* Create a plan for `nshots` independent shots on one velocity model.
- * ndim 2 (nz ignored) or 3; n >= 3 per axis and n >= 2*halo+2 per axis
- * (the two CPML layers of an axis must not overlap); halo >= 0.
+ * ndim 2 (nz ignored) or 3; n >= 3 and n >= halo+2 per axis; halo >= 0.
+ * With n < 2*halo+2 the two CPML slabs of an axis overlap; a node in both
+ * takes the low side's terms, as the reference's if/elseif does
+ * (src/shared/acoustic_2D_xPU.jl:45-56).
  * dtype AWCPML_F64 reproduces the reference's Float64 arithmetic operation
  * for operation; AWCPML_F32 keeps state and arithmetic in FP32.
  * Replaces the array allocation of src/shared/acoustic_2D_xPU.jl:208-215
@@ -187,6 +189,13 @@ int awcpml_get_time_block(const awcpml_plan *plan, int *steps, int *block_rows,
  * support, no reference counterpart. */
 int awcpml_time_stencil2(awcpml_plan *plan, int nlaunches, double *ms_total);
 
+/* Force the kernels' 64-bit address offsets on any grid (enable != 0).
+ * The library switches to them by itself for fields of 4 GiB and more per
+ * shot; results are bit-identical either way.  Two-step launches do not
+ * apply to such plans (they take single steps).  Test support for the
+ * large-grid code path; no reference counterpart. */
+int awcpml_set_wide_offsets(awcpml_plan *plan, int enable);
+
 /* Capture the time loop in HIP graphs: with enable != 0, awcpml_advance and
  * awcpml_time_advance record each (first step, number of steps) sequence
  * of launches once and replay it afterwards, so a solve after
@@ -272,6 +281,16 @@ int awcpml_checkpoint_run(awcpml_plan *plan, int64_t every);
  * gradient (those nodes do not depend on fact).  Leaves the plan's forward
  * state at an intermediate step; the traces are unchanged. */
 int awcpml_adjoint(awcpml_plan *plan, const double *trace_bar, int64_t count, double *fact_bar);
+
+/* Precision of the gradient sum of awcpml_adjoint.  Each reverse step adds
+ * lam * q to a per-shot partial sum in the plan's precision; every `steps`
+ * reverse steps (and at the end of every checkpoint segment) the partial
+ * sums of all shots are added, in shot order, to an FP64 total and reset.
+ * steps = 0 (the default): once per checkpoint segment.  steps = -1: never;
+ * the whole sum over time and shots stays in the plan's precision (the
+ * cheapest, and for FP32 plans on long runs the least accurate).  Extension
+ * beyond the reference, which has no gradient (README.md:297). */
+int awcpml_set_gradient_fold(awcpml_plan *plan, int64_t steps);
 
 /* Message of the calling thread's last failed call ("" if none). */
 const char *awcpml_last_error(void);

@@ -38,6 +38,9 @@
  * reference.
  */
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -590,6 +593,22 @@ void oracle_step(oracle_t *o, long it)
     o->p[1] = o->p[2];
     o->p[2] = t0;
     o->it_done = it;
+}
+
+/* OpenMP threads of the following calls (n <= 0: the runtime's default,
+ * OMP_NUM_THREADS or all cores); returns the count now in effect.  For the
+ * CPU baseline's one-core figure; results do not depend on it. */
+int oracle_set_threads(int n)
+{
+#ifdef _OPENMP
+    static int dflt = 0;
+    if (!dflt) dflt = omp_get_max_threads();
+    omp_set_num_threads(n > 0 ? n : dflt);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
 }
 
 /* Run steps it_done+1 .. min(it_done+nsteps, nt). Returns steps run. */

@@ -85,6 +85,8 @@ def lib():
         L.oracle_set_fact.argtypes = [ctypes.c_void_p, _dp]
         L.oracle_fwi_gradient.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
         L.oracle_fwi_gradient.restype = ctypes.c_int
+        L.oracle_set_threads.argtypes = [ctypes.c_int]
+        L.oracle_set_threads.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -99,6 +101,12 @@ def _f64(a):
 
 class OracleError(AssertionError):
     """Raised for the reference's assertion failures (AssertionError)."""
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle (n <= 0: the default); returns the
+    count in effect."""
+    return lib().oracle_set_threads(int(n))
 
 
 def calc_kab_cpml(halo, dt, npower, d0, alpha_max, K_max, onwhere):

@@ -126,6 +126,33 @@ def small3d(n=(37, 29, 23), halo=5, freetop=True, nt=120, seed=7, nsrc=1):
                 freetop=freetop, ppw=2.0, nt=nt)
 
 
+def overlap_case(n, halo, freetop=False, nt=150, seed=5):
+    """Tiny heterogeneous 2D or 3D case for axes with n < 2*halo + 2, where
+    the two CPML slabs overlap and the low side's branch wins (the
+    reference's if/elseif, acoustic_2D_xPU.jl:45-56); n >= halo + 2."""
+    nd = len(n)
+    dx = 10.0
+    rng = np.random.Generator(np.random.PCG64(seed))
+    vel = np.asfortranarray(1800.0 + 400.0 * rng.random(n))
+    L = [(v - 1) * dx for v in n]
+    srcs = [[(v // 2) * dx for v in n], [(1 + (3 * a) % (v - 2)) * dx for a, v in enumerate(n)]]
+    recs = [[(1 + (7 * r + 3 * a) % (v - 2)) * dx for a, v in enumerate(n)] for r in range(6)]
+    recs.append([0.0] * nd)  # a Dirichlet corner node
+    return dict(n=list(n), l=L, lt=lt_for_nt(nd, (dx,) * nd, float(vel.max()), nt), vel=vel,
+                srcpos=np.array(srcs), t0s=[0.02, 0.03], f0=25.0, recpos=np.array(recs),
+                halo=halo, rcoef=1e-4, freetop=freetop, ppw=2.0, nt=nt)
+
+
+# (n, halo, freetop): overlapping slabs on some axes, including n = halo + 2
+# (no node takes the high side) and n = 2*halo + 2 (the slabs touch)
+OVERLAP_CASES = [
+    ((30, 33), 20, False),
+    ((22, 40), 20, True),
+    ((24, 23, 26), 12, False),
+    ((30, 21, 19), 12, True),
+]
+
+
 def small2d(n=(57, 43), halo=6, freetop=True, nt=300, seed=3):
     dx = 10.0
     d = (dx, dx)

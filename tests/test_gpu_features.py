@@ -11,7 +11,7 @@ import pytest
 import awcpml
 from awcpml import _lib
 from awcpml.autotune import two_step_batch_candidates
-from cases import oracle_problem, rel_l2, small2d, small3d
+from cases import OVERLAP_CASES, oracle_problem, overlap_case, rel_l2, small2d, small3d
 
 pytestmark = pytest.mark.gpu
 
@@ -243,3 +243,135 @@ def test_autotune_batch_keeps_results():
         assert np.array_equal(P.traces(), o.traces())
         P.close()
     B.close()
+
+
+@pytest.mark.parametrize("two", [False, True])
+def test_batch_follows_plan_reconfiguration(two):
+    """A plan of a batch gets new sources (more of them, and another nt) and
+    its traces are read, which replaces its device source/receiver buffers;
+    the batch must notice: advancing without a reset raises, and after a
+    reset the batch rebuilds its argument tables and matches the oracle."""
+    cases = [small3d(n=(37, 29, 23), halo=5, nt=70, seed=1),
+             small3d(n=(45, 41, 33), halo=6, nt=95, seed=2, freetop=False, nsrc=2)]
+    print(f"[gpu] batch reconfiguration two={two}", flush=True)
+    oracles = [oracle_for(c) for c in cases]
+    plans = [plan_from_oracle(o) for o in oracles]
+    if two:
+        for P in plans:
+            P.set_time_block(2, 4, 6, 2)
+    B = awcpml.Batch(plans)
+    B.set_graph(True)
+    B.run()
+    for P, o in zip(plans, oracles):
+        o.run()
+        assert np.array_equal(P.traces(), o.traces())
+    # new acquisition for plan 0: three sources and a longer run
+    c2 = small3d(n=(37, 29, 23), halo=5, nt=110, seed=1, nsrc=3)
+    o2 = oracle_for(c2)
+    o2.run()
+    ps, pr = o2.positions()
+    plans[0].set_sources(0, ps, o2.dt2srctf())
+    plans[0].set_receivers(0, pr)
+    plans[0].traces()  # uploads the new buffers, frees the old ones
+    with pytest.raises(_lib.AwcpmlError, match="reconfigured"):
+        B.advance(1)
+    for _ in range(2):
+        B.run()
+        assert np.array_equal(plans[0].traces(), o2.traces())
+        assert np.array_equal(plans[0].field(), o2.pend())
+        assert np.array_equal(plans[1].traces(), oracles[1].traces())
+    B.close()
+    for P in plans:
+        P.close()
+
+
+@pytest.mark.parametrize("n,halo,freetop", OVERLAP_CASES)
+def test_overlapping_slabs_bitwise(n, halo, freetop):
+    """n < 2*halo+2 on some axes: the two CPML slabs overlap and the low
+    side wins, as the reference's if/elseif (acoustic_2D_xPU.jl:45-56).
+    Every single-step tiling and the two-step launches match the oracle."""
+    case = overlap_case(n, halo, freetop=freetop)
+    print(f"[gpu] overlapping slabs n={n} halo={halo} freetop={freetop}", flush=True)
+    o = oracle_for(case)
+    o.run()
+    P = plan_from_oracle(o)
+    for by, ml, pf in ((4, 7, 2), (1, 16, 3), (2, 40, 5), (1, 3, 1)):
+        P.set_tiling(by, ml, pf)
+        P.reset()
+        P.advance(o.nt)
+        assert np.array_equal(P.traces(), o.traces()), (by, ml, pf)
+        assert np.array_equal(P.field(), o.pend()), (by, ml, pf)
+    for rows, ml in ((4, 5), (8, 64)):
+        P.set_time_block(2, rows, ml, 2)
+        assert P.time_block()[0] == 2
+        P.reset()
+        P.advance(o.nt)
+        assert np.array_equal(P.traces(), o.traces()), ("two", rows, ml)
+        assert np.array_equal(P.field(), o.pend()), ("two", rows, ml)
+    P.close()
+    # FP32 against the FP64 oracle
+    P = plan_from_oracle(o, dtype="f32")
+    P.reset()
+    P.advance(o.nt)
+    assert rel_l2(P.traces(), o.traces()) <= 1e-5
+    P.close()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_wide_offsets_bitwise(dim):
+    """The 64-bit-offset kernels (taken automatically for fields of 4 GiB
+    and more per shot) forced on small grids: bitwise the oracle, alone and
+    in a batch; two-step launches fall back to single steps."""
+    cases = ([small3d(freetop=True, nsrc=2, nt=80), small3d(n=(45, 41, 33), halo=6, nt=60, seed=2,
+                                                           freetop=False)] if dim == 3 else
+             [small2d(nt=150), small2d(n=(61, 47), halo=7, nt=120, freetop=False, seed=9)])
+    print(f"[gpu] wide offsets {dim}D", flush=True)
+    oracles = [oracle_for(c) for c in cases]
+    for o in oracles:
+        o.run()
+    P = plan_from_oracle(oracles[0])
+    P.set_wide_offsets(True)
+    P.set_time_block(2, 4, 8, 2)
+    assert P.time_block()[0] == 1
+    for pf in (1, 2, 3):
+        P.set_tiling(2, 9, pf)
+        P.reset()
+        P.advance(oracles[0].nt)
+        assert np.array_equal(P.traces(), oracles[0].traces()), pf
+        assert np.array_equal(P.field(), oracles[0].pend()), pf
+    P.set_tiling(4, 0, 2)
+    plans = [P] + [plan_from_oracle(o) for o in oracles[1:]]
+    for Q in plans[1:]:
+        Q.set_tiling(4, 0, 2)
+    B = awcpml.Batch(plans)
+    B.run()
+    for Q, o in zip(plans, oracles):
+        assert np.array_equal(Q.traces(), o.traces())
+        assert np.array_equal(Q.field(), o.pend())
+    B.close()
+    for Q in plans:
+        Q.close()
+
+
+def test_many_sources_fall_back_to_single_steps():
+    """A shot with more sources than a two-step launch stages (32) takes
+    single steps, bitwise; 32 sources still take two-step launches."""
+    for nsrc, steps in ((33, 1), (32, 2)):
+        case = small3d(n=(41, 37, 35), halo=5, nt=60, freetop=False)
+        rng = np.random.Generator(np.random.PCG64(nsrc))
+        pts = [[(6 + int(rng.integers(0, 29))) * 10.0, (6 + int(rng.integers(0, 25))) * 10.0,
+                (6 + int(rng.integers(0, 23))) * 10.0] for _ in range(nsrc)]
+        pts[1] = pts[0]  # two sources on one node, added in order
+        case["srcpos"] = np.array(pts)
+        case["t0s"] = [0.02 + 0.001 * s for s in range(nsrc)]
+        print(f"[gpu] {nsrc} sources per shot", flush=True)
+        o = oracle_for(case)
+        o.run()
+        P = plan_from_oracle(o)
+        P.set_time_block(2, 4, 10, 2)
+        P.reset()
+        P.advance(o.nt)
+        assert P.time_block()[0] == steps  # known once the sources are uploaded
+        assert np.array_equal(P.traces(), o.traces()), nsrc
+        assert np.array_equal(P.field(), o.pend()), nsrc
+        P.close()

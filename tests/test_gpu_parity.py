@@ -171,3 +171,34 @@ def test_x_slab_paths_bitwise(name):
         assert np.array_equal(P.traces(), o.traces()), (by, ml, pf)
         assert np.array_equal(P.field(), o.pend()), (by, ml, pf)
     P.close()
+
+
+def test_do_save_snapshots_match_oracle(tmp_path):
+    """solve3D(do_save=True, nsave=k): a snapshot after every k-th step
+    (acoustic_3D_xPU.jl:480-498), whose pcur is bitwise the oracle's field
+    at that step."""
+    case = small3d(freetop=True, nsrc=2, nt=47)
+    nsave = 10
+    print("[gpu] do_save snapshots small3d", flush=True)
+    srcs = awcpml.Sources(case["srcpos"], case["t0s"], [awcpml.rickersource1D] * len(case["t0s"]),
+                          case["f0"])
+    recs = awcpml.Receivers(case["recpos"])
+    pend = awcpml.solve3D(case["l"][0], case["l"][1], case["l"][2], case["lt"], case["vel"], srcs,
+                          recs, halo=case["halo"], rcoef=case["rcoef"], ppw=case["ppw"],
+                          freetop=case["freetop"], do_save=True, nsave=nsave, save_name="snap",
+                          save_dir=str(tmp_path))
+    o = oracle_problem(case)
+    assert o.nt == 47
+    ps, pr = o.positions()
+    saved = sorted(p.name for p in tmp_path.iterdir())
+    assert saved == [f"snap_it{it}.npz" for it in (10, 20, 30, 40)]
+    for it in (10, 20, 30, 40):
+        o.run(nsave)
+        with np.load(tmp_path / f"snap_it{it}.npz", allow_pickle=False) as z:
+            assert np.array_equal(z["pcur"], o.pend()), it
+            assert float(z["lx"]) == case["l"][0] and float(z["lz"]) == case["l"][2]
+            assert int(z["halo"]) == case["halo"]
+            assert np.array_equal(z["possrcs"], ps) and np.array_equal(z["posrecs"], pr)
+    o.run()
+    assert np.array_equal(pend, o.pend())
+    assert np.array_equal(recs.seismograms, o.traces())

@@ -98,8 +98,10 @@ def test_version_and_error_without_gpu_work():
     h = ctypes.c_void_p()
     rc = L.awcpml_plan_create(4, 10, 10, 10, 2, 0, 1, ctypes.byref(h))
     assert rc == _lib.ERR_ARG and b"ndim" in L.awcpml_last_error()
-    rc = L.awcpml_plan_create(3, 10, 10, 10, 5, 0, 1, ctypes.byref(h))
-    assert rc == _lib.ERR_ARG and b"overlap" in L.awcpml_last_error()
+    # overlapping slabs are accepted (left side wins), but the right psi
+    # layers must start inside the grid: n >= halo + 2
+    rc = L.awcpml_plan_create(3, 10, 10, 10, 9, 0, 1, ctypes.byref(h))
+    assert rc == _lib.ERR_ARG and b"needs >= 11" in L.awcpml_last_error()
     rc = L.awcpml_plan_create(2, 10, 10, 1, 1, 7, 1, ctypes.byref(h))
     assert rc == _lib.ERR_ARG and b"dtype" in L.awcpml_last_error()
     assert L.awcpml_reset(None) == _lib.ERR_ARG
@@ -110,3 +112,70 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "libawcpml.so"))
     with pytest.raises(RuntimeError, match="not found"):
         _lib.load()
+
+
+def test_fwi_backward_after_a_later_forward_raises():
+    """fwi.Problem keeps one set of checkpoints: a backward whose forward was
+    followed by another forward on the same problem raises instead of
+    returning a gradient that mixes two models (host logic; the plan is a
+    stand-in that records calls)."""
+    import types
+
+    import torch
+
+    from awcpml import fwi
+
+    class StandInPlan:
+        nshots = 1
+
+        def __init__(self):
+            self.models = []
+
+        def set_model(self, fact, inv_d, inv_d2):
+            self.models.append(fact.copy())
+
+        def checkpoint_run(self, every):
+            pass
+
+        def traces(self, shot):
+            return np.full((3, 2), float(len(self.models)))
+
+        def adjoint(self, trace_bar):
+            return np.full((4, 4), float(len(self.models)))
+
+    pr = object.__new__(fwi.Problem)
+    pr.S = types.SimpleNamespace(inv_d=np.ones(2), inv_d2=np.ones(2))
+    pr.dt, pr.every, pr.forward_count, pr.plan = 1e-3, 0, 0, StandInPlan()
+    _, run1 = pr.forward(np.ones((4, 4)))
+    _, run2 = pr.forward(2 * np.ones((4, 4)))
+    with pytest.raises(RuntimeError, match="stale backward"):
+        pr.pullback(run1, [np.zeros((3, 2))])
+    assert np.all(pr.pullback(run2, [np.zeros((3, 2))]) == 2.0)
+    v = torch.ones((4, 4), dtype=torch.float64, requires_grad=True)
+    t1 = fwi.solve_traces(v, pr)
+    t2 = fwi.solve_traces(v, pr)
+    with pytest.raises(RuntimeError, match="stale backward"):
+        t1.sum().backward()
+    t2.sum().backward()
+    assert v.grad is not None and torch.all(v.grad != 0)
+
+
+def test_snapshot_file_layout(tmp_path):
+    """do_save writes the reference's seven datasets (acoustic_3D_xPU.jl:
+    480-498) with their names, element types, shapes and index order."""
+    import types
+
+    from awcpml.solver import _save_snapshot
+
+    pcur = np.asfortranarray(np.arange(4 * 3 * 2, dtype=np.float64).reshape((4, 3, 2), order="F"))
+    S = types.SimpleNamespace(halo=5, possrcs=np.array([[2, 3, 1]]),
+                              posrecs=np.array([[1, 1, 1], [4, 3, 2]]))
+    (tmp_path / "snap_it10.npz").write_bytes(b"stale")  # replaced, as the reference rm's it
+    path = _save_snapshot(str(tmp_path), "snap", 10, (30.0, 20.0, 10.0), S, pcur)
+    assert path.endswith("snap_it10.npz")
+    with np.load(path, allow_pickle=False) as z:
+        assert sorted(z.files) == sorted(["lx", "ly", "lz", "halo", "pcur", "possrcs", "posrecs"])
+        assert z["lx"].dtype == np.float64 and z["lx"].shape == () and float(z["lz"]) == 10.0
+        assert z["halo"].dtype == np.int64 and int(z["halo"]) == 5
+        assert z["pcur"].shape == (4, 3, 2) and z["pcur"][3, 2, 1] == pcur[3, 2, 1]
+        assert z["possrcs"].dtype == np.int64 and z["posrecs"].shape == (2, 3)

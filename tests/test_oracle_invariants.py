@@ -10,7 +10,8 @@
   of the axes agrees to rounding (the z term is added last).
 * 3D and 2D: a vectorised numpy restatement (tests/numpy_step.py) agrees
   with the C oracle bit for bit on tiny heterogeneous grids, including
-  Dirichlet-node sources/receivers and duplicate sources.
+  Dirichlet-node sources/receivers and duplicate sources, and on grids
+  whose two CPML slabs of an axis overlap (n < 2*halo+2).
 * free top: psi_y/xi_y of the top side stay exactly zero.
 * halo = 0: no CPML term at all (the textbook leapfrog with Dirichlet
   edges), checked against a direct numpy leapfrog.
@@ -23,7 +24,7 @@ import pytest
 
 import oracle
 import numpy_step
-from cases import oracle_problem, rel_l2, small2d, small3d
+from cases import OVERLAP_CASES, oracle_problem, overlap_case, rel_l2, small2d, small3d
 
 
 def test_calc_kab_invariants():
@@ -145,7 +146,20 @@ def test_cyclic_axes_3d():
     (small2d, dict(freetop=False, n=(31, 29), halo=4)),
 ])
 def test_numpy_restatement_bitwise(case_fn, kw):
-    case = case_fn(**kw)
+    _numpy_vs_oracle(case_fn(**kw))
+
+
+@pytest.mark.parametrize("n,halo,freetop", OVERLAP_CASES)
+def test_overlapping_slabs_bitwise(n, halo, freetop):
+    """Grids with n < 2*halo+2: the numpy restatement (boolean masks
+    left = q <= halo, right = not left and q >= n-halo-1) and the oracle's
+    if/elseif agree bit for bit, and the overlap region gets damped."""
+    case = overlap_case(n, halo, freetop=freetop)
+    o = _numpy_vs_oracle(case)
+    assert np.abs(o.xi(0, 0)).max() > 0
+
+
+def _numpy_vs_oracle(case):
     o = oracle_problem(case)
     spacing = [case["l"][a] / (case["n"][a] - 1) for a in range(len(case["n"]))]
     st = numpy_step.from_oracle(o, spacing)
@@ -154,6 +168,8 @@ def test_numpy_restatement_bitwise(case_fn, kw):
     o.run()
     assert np.array_equal(st["pcur"], o.pend())
     assert np.array_equal(st["traces"], o.traces())
+    assert np.abs(o.traces()).max() > 0
+    return o
 
 
 def test_duplicate_and_boundary_sources_bitwise():
