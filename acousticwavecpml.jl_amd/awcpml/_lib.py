@@ -69,6 +69,11 @@ SIGNATURES = {
     "awcpml_checkpoint_run": (ctypes.c_int, [_vp, ctypes.c_int64]),
     "awcpml_adjoint": (ctypes.c_int, [_vp, _dp, ctypes.c_int64, _dp]),
     "awcpml_set_gradient_fold": (ctypes.c_int, [_vp, ctypes.c_int64]),
+    "awcpml_run_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
+                                          ctypes.c_int64, ctypes.c_int, ctypes.c_int, _vp, _dp,
+                                          _dp, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_int),
+                                          ctypes.c_int, _i64p, ctypes.c_int64, _vp, ctypes.c_int,
+                                          _i64p, _vp, _vp, _vp]),
     "awcpml_last_error": (ctypes.c_char_p, []),
     "awcpml_version": (ctypes.c_char_p, []),
 }

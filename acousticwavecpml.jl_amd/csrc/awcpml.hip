@@ -21,6 +21,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2311,6 +2312,215 @@ int awcpml_adjoint(awcpml_plan* P, const double* trace_bar, int64_t count, doubl
                                 : adjoint_t<float>(P, trace_bar, fact_bar);
     P->q_cur = nullptr;
     return st;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------
+// Allocation-free one-shot solve (awcpml_run_noalloc).  The wavefields,
+// psi/xi and the source/receiver tables live in `scratch`, a file-scope
+// device array of AWCPML_NOALLOC_SCRATCH_BYTES; the model, coefficients,
+// wavelets, traces and final field are the caller's device buffers.  The
+// steps are the plan path's own single-step launches (do_step on a plan
+// description that owns nothing), so the results are those of awcpml_run
+// on a one-shot plan with one step per launch.
+// ---------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[AWCPML_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxScratchDevices] = {};  // scratch's address on each device
+
+// The current device's copy of `scratch`, looked up on the first call on
+// that device (with it current).  In host code the array's name is only a
+// stand-in for the symbol; every use goes through this device address.
+int scratch_address(void** out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices)
+        return fail(AWCPML_ERR_ARG, "device %d: the scratch table holds %d devices", dev, kMaxScratchDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* ptr = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(AWCPML_ERR_HIP, "hipGetSymbolAddress gave no address for the scratch array");
+        g_scratch_dev[dev] = ptr;
+    }
+    *out = g_scratch_dev[dev];
+    return AWCPML_OK;
+}
+
+// Zero n 16-byte words (a kernel rather than hipMemsetAsync, so that a
+// caller may capture the call in a graph).
+__global__ void zero_words(uint4* p, size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride)
+        p[q] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// Up to 256 values per launch, carried in the kernel argument (no host
+// memory is referenced after the launch is enqueued).
+struct Put64 {
+    int64_t v[256];
+};
+__global__ void put_i64(int64_t* dst, int count, Put64 a) {
+    if ((int)threadIdx.x < count) dst[threadIdx.x] = a.v[threadIdx.x];
+}
+
+// CSR bounds of the one shot: src_start = {0, nsrcs}, src_seq = 1 (the
+// sources are added by one thread in source order, exact for any number of
+// sources sharing a node), rec_start = {0, nrecs}.
+__global__ void put_io_bounds(int* meta, int nsrcs, int nrecs) {
+    if (threadIdx.x == 0) {
+        meta[0] = 0;
+        meta[1] = nsrcs;
+        meta[2] = 1;
+        meta[3] = 0;
+        meta[4] = nrecs;
+    }
+}
+
+size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+
+// Linear element offsets of `count` 1-based positions into dst (device).
+int upload_lin(const awcpml_plan& P, const int64_t* pos, int count, int64_t* dst, hipStream_t s) {
+    for (int c0 = 0; c0 < count; c0 += 256) {
+        Put64 a;
+        const int cnt = std::min(256, count - c0);
+        for (int q = 0; q < cnt; ++q) a.v[q] = lin_index(&P, pos, count, c0 + q);
+        put_i64<<<dim3(1), dim3(256), 0, s>>>(dst + c0, cnt, a);
+        HIP_TRY(hipGetLastError());
+    }
+    return AWCPML_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int awcpml_run_noalloc(int ndim, int64_t nx, int64_t ny, int64_t nz, int halo, int dtype,
+                       const void* fact, const double* inv_d, const double* inv_d2,
+                       const void* const* coef, const int* active, int nsrcs,
+                       const int64_t* srcpos, int64_t nt, const void* dt2srctf, int nrecs,
+                       const int64_t* recpos, void* traces, void* pend, void* stream) {
+    g_err.clear();
+    if (ndim != 2 && ndim != 3) return fail(AWCPML_ERR_ARG, "ndim must be 2 or 3 (got %d)", ndim);
+    if (dtype != AWCPML_F64 && dtype != AWCPML_F32) return fail(AWCPML_ERR_ARG, "unknown dtype %d", dtype);
+    if (halo < 0) return fail(AWCPML_ERR_ARG, "halo must be >= 0");
+    if (nsrcs < 0 || nrecs < 0 || nt < 0) return fail(AWCPML_ERR_ARG, "negative count");
+    if (ndim == 2) nz = 1;
+    const int64_t n[3] = {nx, ny, nz};
+    for (int a = 0; a < ndim; ++a)
+        if (n[a] < 3 || n[a] < (int64_t)halo + 2)
+            return fail(AWCPML_ERR_ARG, "axis %d has %lld points (need >= 3 and >= halo+2 = %d)", a,
+                        (long long)n[a], halo + 2);
+    if (!fact || !inv_d || !inv_d2) return fail(AWCPML_ERR_ARG, "NULL model argument");
+    if (halo > 0 && !coef) return fail(AWCPML_ERR_ARG, "coef is NULL");
+    if (nsrcs > 0 && (!srcpos || (nt > 0 && !dt2srctf))) return fail(AWCPML_ERR_ARG, "NULL source argument");
+    if (nrecs > 0 && (!recpos || (nt > 0 && !traces))) return fail(AWCPML_ERR_ARG, "NULL receiver argument");
+
+    // a plan description that owns nothing: every pointer is the caller's
+    // or inside scratch (its destructor frees nothing)
+    awcpml_plan P;
+    P.ndim = ndim;
+    P.dtype = dtype;
+    P.nshots = 1;
+    P.halo = halo;
+    for (int a = 0; a < 3; ++a) P.n[a] = n[a];
+    P.esize = dtype == AWCPML_F64 ? 8 : 4;
+    P.px = nx;  // the caller's fact is dense, so the fields are too
+    P.field_elems = nx * ny * nz;
+    P.stream = (hipStream_t)stream;
+    int st;
+    if ((st = check_positions(&P, srcpos, nsrcs, "source"))) return st;
+    if ((st = check_positions(&P, recpos, nrecs, "receiver"))) return st;
+
+    // scratch layout: io bounds, source and receiver offsets, 3 fields,
+    // psi (2 copies) and xi per absorbing side, each 256-byte aligned
+    const size_t fbytes = (size_t)P.field_elems * P.esize;
+    size_t off = 0;
+    const size_t o_meta = off;
+    off += 256;
+    const size_t o_src = off;
+    off += align256((size_t)nsrcs * 8);
+    const size_t o_rec = off;
+    off += align256((size_t)nrecs * 8);
+    const size_t o_state = off;
+    size_t o_p[3];
+    for (int b = 0; b < 3; ++b) {
+        o_p[b] = off;
+        off += align256(fbytes);
+    }
+    size_t o_psi[3][2][2] = {}, o_xi[3][2] = {};
+    for (int a = 0; a < ndim; ++a) {
+        P.psi_elems[a] = slab_elems(&P, a, halo + 1);
+        P.xi_elems[a] = slab_elems(&P, a, halo);
+        for (int s = 0; s < 2; ++s) {
+            for (int c = 0; c < 2; ++c) {
+                o_psi[a][s][c] = off;
+                off += align256((size_t)P.psi_elems[a] * P.esize);
+            }
+            o_xi[a][s] = off;
+            off += align256((size_t)P.xi_elems[a] * P.esize);
+        }
+    }
+    if (off > (size_t)AWCPML_NOALLOC_SCRATCH_BYTES)
+        return fail(AWCPML_ERR_ARG,
+                    "this problem needs %zu bytes of scratch; the library's allocation-free array holds "
+                    "%zu (use a plan, awcpml_plan_create, for larger grids)",
+                    off, (size_t)AWCPML_NOALLOC_SCRATCH_BYTES);
+
+    void* base = nullptr;
+    if ((st = scratch_address(&base))) return st;
+    char* sc = (char*)base;
+    for (int b = 0; b < 3; ++b) P.p[b] = sc + o_p[b];
+    P.fact = const_cast<void*>(fact);
+    for (int a = 0; a < ndim; ++a) {
+        P.inv_d[a] = inv_d[a];
+        P.inv_d2[a] = inv_d2[a];
+        for (int s = 0; s < 2; ++s) {
+            P.psi[a][s][0] = sc + o_psi[a][s][0];
+            P.psi[a][s][1] = sc + o_psi[a][s][1];
+            P.xi[a][s] = sc + o_xi[a][s];
+            P.active[a][s] = halo > 0 && (!active || active[2 * a + s]) ? 1 : 0;
+        }
+        for (int c = 0; c < 8; ++c) {
+            P.coef[a][c] = halo > 0 ? const_cast<void*>(coef[8 * a + c]) : nullptr;
+            if (halo > 0 && !P.coef[a][c])
+                return fail(AWCPML_ERR_ARG, "coefficient vector %d of axis %d is NULL", c, a);
+        }
+    }
+    int* meta = (int*)(sc + o_meta);
+    P.d_src_lin = sc + o_src;
+    P.d_src_wav = const_cast<void*>(dt2srctf);
+    P.d_src_start = meta;
+    P.d_src_seq = meta + 2;
+    P.d_rec_lin = sc + o_rec;
+    P.d_rec_start = meta + 3;
+    P.d_traces = traces;
+    P.nt = nt;
+    P.n_src_total = nsrcs;
+    P.n_rec_total = nrecs;
+    P.has_io = nsrcs + nrecs > 0;
+    P.io_dirty = false;
+    if ((st = default_tiling(&P))) return st;
+
+    hipStream_t s = P.stream;
+    put_io_bounds<<<dim3(1), dim3(64), 0, s>>>(meta, nsrcs, nrecs);
+    HIP_TRY(hipGetLastError());
+    if ((st = upload_lin(P, srcpos, nsrcs, (int64_t*)P.d_src_lin, s))) return st;
+    if ((st = upload_lin(P, recpos, nrecs, (int64_t*)P.d_rec_lin, s))) return st;
+    // the previous call's state: fields, psi and xi start at zero
+    zero_words<<<dim3(1024), dim3(256), 0, s>>>((uint4*)(sc + o_state), (off - o_state) / 16);
+    HIP_TRY(hipGetLastError());
+    for (int64_t it = 0; it < nt; ++it)
+        if ((st = do_step(&P, it))) return st;
+    if (pend)
+        HIP_TRY(hipMemcpyAsync(pend, P.p[(P.rot + 1) % 3], fbytes, hipMemcpyDeviceToDevice, s));
+    if (!stream) HIP_TRY(hipStreamSynchronize(nullptr));
+    return AWCPML_OK;
 }
 
 }  // extern "C"

@@ -282,6 +282,52 @@ int awcpml_checkpoint_run(awcpml_plan *plan, int64_t every);
  * state at an intermediate step; the traces are unchanged. */
 int awcpml_adjoint(awcpml_plan *plan, const double *trace_bar, int64_t count, double *fact_bar);
 
+/* ------------------------------------------------------------------ */
+/* Allocation-free solve.                                               */
+/* ------------------------------------------------------------------ */
+
+/* Bytes of the library's static device scratch array, one copy per GPU it
+ * runs on, held until the process ends. */
+#define AWCPML_NOALLOC_SCRATCH_BYTES (64ull << 20)
+
+/* awcpml_run for one shot without any device allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The wavefields, psi/xi and the source/receiver tables live
+ * in the library's static scratch array; a problem needing more than
+ * AWCPML_NOALLOC_SCRATCH_BYTES of it (3 fields plus the CPML slabs; with
+ * halo 20: 2D up to 1631^2 in FP64 or 2323^2 in FP32, 3D up to 109^3 in
+ * FP64 or 144^3 in FP32) fails with AWCPML_ERR_ARG; larger problems need a
+ * plan (awcpml_plan_create).
+ *
+ * Device inputs, in the precision of `dtype` (8 or 4 bytes per value):
+ *   fact      nx*ny*nz values, x fastest, no padding;
+ *   coef      host array of ndim*8 device pointers: per axis a_l, a_r,
+ *             b_K_l, b_K_r (halo values), a_hl, a_hr, b_K_hl, b_K_hr
+ *             (halo+1 values), as for awcpml_set_cpml (unused if halo = 0);
+ *   dt2srctf  nt x nsrcs values, column-major.
+ * Host inputs: inv_d, inv_d2 (ndim FP64 values each); srcpos (nsrcs x ndim)
+ * and recpos (nrecs x ndim), 1-based, column-major, as for
+ * awcpml_set_sources / awcpml_set_receivers; active: ndim x 2 flags (axis
+ * major) marking the CPML sides that absorb, or NULL for all of them
+ * (a side whose coefficients are a = 0, b_K = 1 may be marked 0 to skip it,
+ * as awcpml_set_cpml does; the results then differ at most in the sign of
+ * zeros).
+ * Device outputs: traces (nt x nrecs values, column-major) and pend (the
+ * final pcur, nx*ny*nz values, or NULL).
+ * stream: a hipStream_t; the call enqueues its work on it and returns
+ * without waiting.  NULL: the null stream, and the call waits for the
+ * result.  The results equal awcpml_run's on a one-shot plan with the same
+ * inputs and one step per launch.  Every call on one GPU uses that GPU's
+ * one scratch array: calls on one stream run in order, calls that overlap
+ * in time on one GPU (other streams, other threads) race and corrupt each
+ * other. */
+int awcpml_run_noalloc(int ndim, int64_t nx, int64_t ny, int64_t nz, int halo, int dtype,
+                       const void *fact, const double *inv_d, const double *inv_d2,
+                       const void *const *coef, const int *active, int nsrcs,
+                       const int64_t *srcpos, int64_t nt, const void *dt2srctf, int nrecs,
+                       const int64_t *recpos, void *traces, void *pend, void *stream);
+
 /* Precision of the gradient sum of awcpml_adjoint.  Each reverse step adds
  * lam * q to a per-shot partial sum in the plan's precision; every `steps`
  * reverse steps (and at the end of every checkpoint segment) the partial
