@@ -253,19 +253,19 @@ __device__ __forceinline__ void two_second(const March<T, Off, 3>& M, const TwoO
 // merged into stores through a selected address, which would keep the
 // whole ring out of registers.
 template <typename T, typename Off, int W, int SA, bool SB, bool GEN>
-__device__ __forceinline__ NodeLoads<T> two_load_node(const March<T, Off, 3>& M, int nz, int k,
-                                                      int mside, int wy) {
+__device__ __forceinline__ NodeLoads<T> two_load_node(const March<T, Off, 3>& M, Off dxm, Off dym,
+                                                      int nz, int k, int mside) {
     NodeLoads<T> q = {};
-    constexpr Off E = (Off)sizeof(T);
     const Off c = M.col + (Off)k * M.sm;
-    const int lane = (int)threadIdx.x;
     if (!GEN || k + 1 <= nz - 1) q.pp = ldg(M.pcur, c + M.sm);
     if (GEN && (k < 1 || k > nz - 2)) return q;
     // issued by every lane and wave (the others reload their own node, a
     // cache hit), so that each march position issues the same loads and the
-    // waits the compiler places for the ring slots stay exact
-    q.xm = ldg(M.pcur, lane == 0 ? c - E : (lane == kTileX - 1 ? c + E : c));
-    q.ym = ldg(M.pcur, wy == 0 ? c - M.sy : (wy == W - 1 ? c + M.sy : c));
+    // waits the compiler places for the ring slots stay exact; dxm / dym:
+    // the lane's (wave's) offset to its x (y) neighbour outside the block,
+    // 0 inside
+    q.xm = ldg(M.pcur, c + dxm);
+    q.ym = ldg(M.pcur, c + dym);
     q.po = ldg_stream(M.pold, c);
     q.f = ldg(M.fact, c);
     if (SA == 2) {
@@ -297,6 +297,8 @@ struct TwoGeom {
     bool in1, own;
     bool inx;     // the lane's x is its clamped x (pcur there is the lane's own column)
     bool iny;     // the wave's y is its clamped y
+    uint32_t dxm; // 3D: byte offset from the lane's node to the x neighbour it loads
+    uint32_t dym; // 3D: byte offset from the wave's row to the y neighbour it loads
 };
 
 // ZM = false: no z slab layer in the chunk.  One code path for both sides
@@ -307,7 +309,7 @@ __device__ __forceinline__ NodeLoads<T> two_load(const TwoGeom& G, const March<T
                                                  int k, int wy) {
     int mz;
     const int zs = ZM ? classify(k, G.nz, G.halo, G.act, &mz) : -1;
-    return two_load_node<T, Off, W, SA, SB, GEN>(M, G.nz, k, zs, wy);
+    return two_load_node<T, Off, W, SA, SB, GEN>(M, (Off)G.dxm, (Off)G.dym, G.nz, k, zs);
 }
 
 template <typename T, typename Off, int SA, bool SB, bool ZM>
@@ -347,6 +349,8 @@ struct TwoCtx {
     T p1m;
     T pz0, pz1;   // pcur of the wave's column at planes k-1 and k
     int slot;
+    // 3D: element offsets of the LDS ring slots of planes k-1, k, k+1
+    int so_p, so_c, so_n;
 };
 
 // The shot's sources for step t+1, staged in LDS at the start of a launch
@@ -380,10 +384,14 @@ __device__ __forceinline__ int two_stage_sources(const Stencil2Args<T>& B, int64
 // refilled, owned plane, step t+2 due), so none is tested.
 template <typename T, typename Off, int W, int PF, int SA, bool SB, bool ZM, bool GEN, int U>
 __device__ __forceinline__ void two_iter(const Stencil2Args<T>& B, const March<T, Off, 3>& M,
-                                         const TwoOut<T, Off>& O, T (*ring)[W][kTileX],
-                                         T (*pring)[W][kTileX], const TwoGeom& G,
+                                         const TwoOut<T, Off>& O, T (*ring)[W + 2][kTileX],
+                                         T (*pring)[W + 2][kTileX], const TwoGeom& G,
                                          const TwoSrc<T>& S, TwoCtx<T, PF>& C, int k) {
     const int lane = (int)threadIdx.x;
+    // this lane's column in the padded LDS rings, one row above its own:
+    // slot c's rows wy-1, wy, wy+1 are at [so], [so + 64], [so + 128]
+    T* const pb = &pring[0][0][0] + G.wy * kTileX + lane;
+    T* const rb = &ring[0][0][0] + G.wy * kTileX + lane;
     Mid<T> dn = {};
     const T pp = own_reg(C.qr[U].pp);
     if (!GEN || (k >= 1 && k <= G.nz - 2)) {
@@ -395,8 +403,12 @@ __device__ __forceinline__ void two_iter(const Stencil2Args<T>& B, const March<T
         const T edge = C.qr[U].xm;
         const T xm = lane == 0 ? edge : sl;
         const T xp = lane == kTileX - 1 ? edge : sr;
-        const T ym = G.wy == 0 ? C.qr[U].ym : pring[C.slot][G.wy - 1 < 0 ? 0 : G.wy - 1][lane];
-        const T yp = G.wy == W - 1 ? C.qr[U].ym : pring[C.slot][G.wy + 1 > W - 1 ? W - 1 : G.wy + 1][lane];
+        // rows wy-1 and wy+1 of the padded ring (rows 0 and W+1 are padding,
+        // read and discarded by the block's outer rows)
+        const T ymr = pb[C.so_c];
+        const T ypr = pb[C.so_c + 2 * kTileX];
+        const T ym = G.wy == 0 ? C.qr[U].ym : ymr;
+        const T yp = G.wy == W - 1 ? C.qr[U].ym : ypr;
         dn = two_first_at<T, Off, SA, SB, ZM>(G, M, k, C.qr[U], C.pz0, pc, xm, xp, ym, yp);
         // inject_sources! of step t+1, in source order
         if (k >= G.ksrc_lo && k <= G.ksrc_hi) {
@@ -413,8 +425,8 @@ __device__ __forceinline__ void two_iter(const Stencil2Args<T>& B, const March<T
     __builtin_amdgcn_sched_barrier(0);
     // this column's pcur at plane k+1 for the neighbouring rows (0 for a
     // Dirichlet row), and p^{t+1} at plane k
-    pring[C.slot == 2 ? 0 : C.slot + 1][G.wy][lane] = G.iny ? pp : T(0);
-    ring[C.slot][G.wy][lane] = dn.p1;
+    pb[C.so_n + kTileX] = G.iny ? pp : T(0);
+    rb[C.so_c + kTileX] = dn.p1;
     if (G.own && (!GEN || (k >= G.kbeg && k < G.kend)))
         stg_stream(O.p1, M.col + (Off)k * M.sm, dn.p1);
     __syncthreads();
@@ -422,16 +434,18 @@ __device__ __forceinline__ void two_iter(const Stencil2Args<T>& B, const March<T
     // block's halo rows, which store nothing (a wave-uniform branch that
     // leaves their issue slots to the other waves)
     if ((!GEN || k - 1 >= G.kbeg) && G.wy >= 1 && G.wy <= W - 2) {
-        const int sp = C.slot == 0 ? 2 : C.slot - 1;  // plane k-1
         const T xm = lane_shift(C.d.p1, true);
         const T xp = lane_shift(C.d.p1, false);
-        const T ym = ring[sp][G.wy - 1][lane];
-        const T yp = ring[sp][G.wy + 1][lane];
+        const T ym = rb[C.so_p];                 // plane k-1, row wy-1
+        const T yp = rb[C.so_p + 2 * kTileX];    // plane k-1, row wy+1
         two_second_at<T, Off, SA, SB, ZM>(G, M, O, C.d, C.p1m, dn.p1, xm, xp, ym, yp, G.own, k - 1);
     }
     C.p1m = C.d.p1;
     C.d = dn;
-    C.slot = C.slot == 2 ? 0 : C.slot + 1;
+    const int so = C.so_p;
+    C.so_p = C.so_c;
+    C.so_c = C.so_n;
+    C.so_n = so;
 }
 
 // Positions k+u, u in Us..., in order, ring slot u % PF (a constant), each
@@ -439,8 +453,8 @@ __device__ __forceinline__ void two_iter(const Stencil2Args<T>& B, const March<T
 template <typename T, typename Off, int W, int PF, int SA, bool SB, bool ZM, bool GEN, bool CHECK,
           int... Us>
 __device__ __forceinline__ void two_group(const Stencil2Args<T>& B, const March<T, Off, 3>& M,
-                                          const TwoOut<T, Off>& O, T (*ring)[W][kTileX],
-                                          T (*pring)[W][kTileX], const TwoGeom& G,
+                                          const TwoOut<T, Off>& O, T (*ring)[W + 2][kTileX],
+                                          T (*pring)[W + 2][kTileX], const TwoGeom& G,
                                           const TwoSrc<T>& S, TwoCtx<T, PF>& C, int k,
                                           std::integer_sequence<int, Us...>) {
     ((!CHECK || k + Us <= G.kend
@@ -459,8 +473,8 @@ __device__ __forceinline__ void two_group(const Stencil2Args<T>& B, const March<
 // kend are block-uniform) and so meets every barrier.
 template <typename T, typename Off, int W, int PF, int SA, bool SB, bool ZM>
 __device__ __forceinline__ void two_march(const Stencil2Args<T>& B, const March<T, Off, 3>& M,
-                                          const TwoOut<T, Off>& O, T (*ring)[W][kTileX],
-                                          T (*pring)[W][kTileX], const TwoGeom& G,
+                                          const TwoOut<T, Off>& O, T (*ring)[W + 2][kTileX],
+                                          T (*pring)[W + 2][kTileX], const TwoGeom& G,
                                           const TwoSrc<T>& S) {
     TwoCtx<T, PF> C;
     C.pz0 = G.kbeg >= 2 ? ldg(M.pcur, M.col + (Off)(G.kbeg - 2) * M.sm) : T(0);
@@ -473,8 +487,14 @@ __device__ __forceinline__ void two_march(const Stencil2Args<T>& B, const March<
     }
     C.d = Mid<T>{};
     C.p1m = T(0);
-    C.slot = (G.kbeg - 1) % 3;
-    pring[C.slot][G.wy][threadIdx.x] = G.iny ? C.pz1 : T(0);
+    {
+        constexpr int SS = (W + 2) * kTileX;  // elements per ring slot
+        const int s0 = (G.kbeg - 1) % 3;
+        C.so_c = s0 * SS;
+        C.so_n = (s0 == 2 ? 0 : s0 + 1) * SS;
+        C.so_p = (s0 == 0 ? 2 : s0 - 1) * SS;
+    }
+    (&pring[0][0][0])[C.so_c + (G.wy + 1) * kTileX + threadIdx.x] = G.iny ? C.pz1 : T(0);
     __syncthreads();
     constexpr int H = (2 + PF - 1) / PF * PF;
     int k = G.kbeg - 1;
@@ -499,8 +519,10 @@ __device__ __forceinline__ void two_march(const Stencil2Args<T>& B, const March<
 // step t+1 (the wavelet sample its sources add).
 template <typename T, typename Off, int W, int PF>
 __device__ __forceinline__ void stencil3d_two_block(const Stencil2Args<T>& B, int64_t it0, int lb) {
-    __shared__ T ring[3][W][kTileX];    // p^{t+1} planes
-    __shared__ T pring[3][W][kTileX];   // pcur planes
+    // three-plane rings of p^{t+1} and pcur rows, padded with one row on
+    // each side: wave wy's row is row wy+1
+    __shared__ T ring[3][W + 2][kTileX];
+    __shared__ T pring[3][W + 2][kTileX];
     __shared__ TwoSrc<T> S;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* lds = reinterpret_cast<T*>(smem_raw);
@@ -595,6 +617,8 @@ __device__ __forceinline__ void stencil3d_two_block(const Stencil2Args<T>& B, in
     G.own = own;
     G.inx = i == ic;
     G.iny = j == jc;
+    G.dxm = lane == 0 ? (uint32_t)(0u - (uint32_t)E) : (lane == kTileX - 1 ? (uint32_t)E : 0u);
+    G.dym = wy == 0 ? (uint32_t)(0u - (uint32_t)M.sy) : (wy == W - 1 ? (uint32_t)M.sy : 0u);
 
     const bool xs = __any(M.sa.in);
     const bool ys = M.sb.in;  // wave-uniform

@@ -396,12 +396,35 @@ def load_traffic(kernel, workload):
     return None
 
 
+def calibration_copy(nbytes=1 << 30, repeats=3):
+    """A plain device-to-device copy of `nbytes` (`repeats` times) through
+    the HIP runtime the library uses: the known-size reference that scales
+    the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
+    from awcpml import _lib
+    _lib.load()
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    hip.hipDeviceSynchronize.argtypes = []
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), nbytes) == 0 and hip.hipMalloc(ctypes.byref(dst), nbytes) == 0
+    assert hip.hipMemset(src, 1, nbytes) == 0
+    for _ in range(repeats):
+        assert hip.hipMemcpy(dst, src, nbytes, 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(f"[profile] calibration copy: {repeats} x {nbytes} bytes read + written", flush=True)
+
+
 def profile_kernel(S, dtype, nshots, tiling, launches, time_block=None):
     """Profiler target: only the stencil launches of one variant at a fixed
     launch shape (no autotuning, so every dispatch of the kernel is the same
-    shape), then a 1 GiB FP32 device copy (1 GiB read + 1 GiB written) that
+    shape), then a 1 GiB device copy (1 GiB read + 1 GiB written) that
     calibrates the FETCH_SIZE / WRITE_SIZE counters."""
-    import torch
     P = build_plan(S, dtype, nshots)
     if tiling:
         P.set_tiling(*tiling[:3])
@@ -420,13 +443,36 @@ def profile_kernel(S, dtype, nshots, tiling, launches, time_block=None):
           f"time_block={P.time_block()} {launches} launches "
           f"{ms:.4f} ms/launch, {B} algorithmic bytes/launch, {B / ms / 1e6:.1f} GB/s", flush=True)
     P.close()
-    n = 1 << 28
-    src = torch.ones(n, dtype=torch.float32, device="cuda")
-    dst = torch.empty_like(src)
-    for _ in range(3):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    print(f"[profile] calibration copy: 3 x {n * 4} bytes read + written", flush=True)
+    calibration_copy()
+
+
+def profile_batch2d(args):
+    """Profiler target: only the stencil launches of the 64-shot 2D workload
+    (1066^2, FP32) at a fixed launch shape (no autotuning), then the 1 GiB
+    calibration copy."""
+    from cases import batch2d
+    case = batch2d()
+    S = _setup(case)
+    P = _shot_plan(S, case["shots"], "f32")
+    tiling = tuple(int(v) for v in args.tiling.split(",")) if args.tiling else (4, 64, 3, 4)
+    P.set_tiling(*tiling[:3])
+    if len(tiling) > 3:
+        P.set_shot_group(tiling[3])
+    steps = 1
+    if args.time_block:
+        P.set_time_block(2, *(int(v) for v in args.time_block.split(",")))
+        steps = 2
+    P.reset()
+    n = args.launches
+    timer = P.time_stencil2 if steps == 2 else P.time_stencil
+    timer(1)
+    ms = timer(n) / n
+    B, _ = algorithmic_bytes(S.n, S.halo, False, 4, P.nshots)
+    print(f"[profile] batch2d FP32 S={P.nshots} tiling={P.tiling()} shot_group={P.shot_group()} "
+          f"steps/launch={steps} {n} launches {ms:.4f} ms/launch, {B * steps} algorithmic bytes/launch, "
+          f"{B * steps / ms / 1e6:.1f} GB/s", flush=True)
+    P.close()
+    calibration_copy()
 
 
 def profile_uneven(args):
@@ -486,6 +532,9 @@ def main():
     ap.add_argument("--time-block", default="",
                     help="block_rows,march_len,prefetch: --profile-kernel times two-step launches")
     ap.add_argument("--launches", type=int, default=20)
+    ap.add_argument("--profile-batch2d", action="store_true",
+                    help="profiler target: stencil launches of the 64-shot 1066^2 2D workload (FP32); "
+                         "--tiling block_y,march_len,prefetch,shot_group, --time-block for two-step")
     ap.add_argument("--profile-fwi", action="store_true",
                     help="profiler target: one FP32 FWI gradient of the 554^2 32-shot workload")
     ap.add_argument("--profile-uneven", action="store_true",
@@ -517,6 +566,10 @@ def main():
         J, g = fwi.misfit_gradient(P, obs, every=fc["every"])
         print(f"[profile] fwi2d FP32 gradient {time.perf_counter() - t0:.3f} s, misfit {J:.4e}", flush=True)
         P.close()
+        return
+
+    if args.profile_batch2d:
+        profile_batch2d(args)
         return
 
     if args.profile_uneven:
