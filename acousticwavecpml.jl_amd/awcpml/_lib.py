@@ -69,6 +69,8 @@ SIGNATURES = {
     "awcpml_checkpoint_run": (ctypes.c_int, [_vp, ctypes.c_int64]),
     "awcpml_adjoint": (ctypes.c_int, [_vp, _dp, ctypes.c_int64, _dp]),
     "awcpml_set_gradient_fold": (ctypes.c_int, [_vp, ctypes.c_int64]),
+    "awcpml_set_adjoint_group": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "awcpml_get_adjoint_group": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "awcpml_run_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
                                           ctypes.c_int64, ctypes.c_int, ctypes.c_int, _vp, _dp,
                                           _dp, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_int),

@@ -170,6 +170,17 @@ class Plan:
         default; -1: never)."""
         _lib.check(_lib.load().awcpml_set_gradient_fold(self._h, int(steps)))
 
+    def set_adjoint_group(self, shots=0):
+        """Shots per thread of the reverse step (0: automatic); the shots
+        of a group share one gradient accumulator."""
+        _lib.check(_lib.load().awcpml_set_adjoint_group(self._h, int(shots)))
+
+    def adjoint_group(self):
+        """Shots per thread the reverse step uses."""
+        v = ctypes.c_int()
+        _lib.check(_lib.load().awcpml_get_adjoint_group(self._h, ctypes.byref(v)))
+        return v.value
+
     def adjoint(self, trace_bar):
         """dJ/dfact summed over the shots, for the trace cotangents
         ``trace_bar`` = dJ/dtraces: one (nt, nrecs) array per shot (a list,

@@ -21,8 +21,9 @@
 // Dirichlet nodes are constants of the forward map, so lam stays zero on
 // them.  alpha^{s+1} is recomputed where it is needed from alpha^{s+2} and
 // the w values around the node, the same fusion the forward kernel uses
-// for psi, so one launch per step reads lam^{s+1}, lam^{s+2}, f, q and
-// the gradient and writes lam^s.  Every absorbing side touches the nodes
+// for psi, so one launch per step reads lam^{s+1}, lam^{s+2} and q of
+// every shot and writes lam^s; f and the gradient accumulator are read
+// once per group of shots.  Every absorbing side touches the nodes
 // 0 <= m <= halo+1 ("extended" range); the plan requires n >= 2 halo + 4
 // on every absorbing axis so the two sides' ranges are disjoint.
 #pragma once
@@ -57,7 +58,7 @@ struct AdjointArgs {
     T* lam0;                  // lam^s (written)
     const T* fact;            // shared by the shots
     const T* q;               // q^s
-    T* grad;                  // per-shot accumulators of fbar
+    T* grad;                  // accumulators of fbar, one per shot group
     T inv_d[3];
     T inv_d2[3];
     AdjSide<T> side[3][2];
@@ -66,8 +67,15 @@ struct AdjointArgs {
     int64_t xi_stride[3];
     int ndim;
     int nshots;
-    int march_len, nchunks;   // march (z in 3D, y in 2D) chunking
-    int tiles_x, tiles_y;     // 64-wide x tiles; 3D: row groups, 2D: chunk groups
+    int group;                // shots per thread (one gradient accumulator per group)
+    int tiles_x, tiles_y;     // 64-wide x tiles, row groups
+    // block classes: per axis (x tiles, row groups, 3D planes) the tiles
+    // none of whose nodes lies in the CPML range of an absorbing side,
+    // [int_lo, int_hi) of ntiles; class c (bit a set: the tiles of axis a
+    // outside that range) holds the launch's logical blocks
+    // [cls_start[c], cls_start[c+1])
+    int ntiles[3], int_lo[3], int_hi[3];
+    int cls_start[9];
     int logical_blocks, grid_blocks;
 };
 
@@ -188,242 +196,205 @@ __device__ __forceinline__ void adj_term(const AdjCoef<T>& C, const AdjVals<T>& 
     acc += (z0 - T(2) * z1 + z2) * inv_d2;
 }
 
-// An axis across the march (x, and y in 3D): its CPML data is the same for
-// every node of a thread's march, and the slab offsets advance by a fixed
-// stride per march step.
+// CPML data of one node on one axis with its slab offsets: ob of the
+// node's own beta layer m-1, oa of alpha edge m-1, ls between neighbouring
+// layers/edges (offsets follow slab_index).
 template <typename T>
 struct AdjAxis {
     AdjCoef<T> C;
-    const T* be_rd;
-    T* be_wr;
-    const T* al_rd;
-    T* al_wr;
-    int64_t ob0, oa0;   // offsets (own beta layer, alpha edge m-1) at march index 0
-    int64_t ls;         // layer stride
-    int64_t sb, sa;     // beta / alpha offset per march step
+    int64_t ob, oa, ls;
 };
 
-// Offsets follow slab_index: x slabs are [march][row][layer] (layer
-// fastest), y slabs (3D) are [z][layer][x].
-template <typename T, int NDIM>
-__device__ __forceinline__ AdjAxis<T> adj_axis(const AdjointArgs<T>& A, int a, int shot,
-                                               int64_t i, int64_t j) {
+template <typename T>
+__device__ __forceinline__ AdjAxis<T> adj_axis(const AdjointArgs<T>& A, int a, int64_t i,
+                                               int64_t j, int64_t k) {
     AdjAxis<T> X;
-    X.C = adj_coef<T>(A, a, a == 0 ? i : j);
-    const int s = X.C.side;
-    X.be_rd = (s ? A.side[a][1].be_rd : A.side[a][0].be_rd) + shot * A.xi_stride[a];
-    X.be_wr = (s ? A.side[a][1].be_wr : A.side[a][0].be_wr) + shot * A.xi_stride[a];
-    X.al_rd = (s ? A.side[a][1].al_rd : A.side[a][0].al_rd) + shot * A.psi_stride[a];
-    X.al_wr = (s ? A.side[a][1].al_wr : A.side[a][0].al_wr) + shot * A.psi_stride[a];
+    const int64_t c = a == 0 ? i : (a == 1 ? j : k);
+    X.C = adj_coef<T>(A, a, c);
     const int64_t h = A.halo, ny = A.n[1], px = A.px, L = X.C.m - 1;
-    if (a == 0) {
-        X.ls = 1;
-        X.ob0 = L + (NDIM == 3 ? h * j : 0);
-        X.oa0 = L + (NDIM == 3 ? (h + 1) * j : 0);
-        X.sb = NDIM == 3 ? h * ny : h;
-        X.sa = NDIM == 3 ? (h + 1) * ny : h + 1;
-    } else {
-        X.ls = px;
-        X.ob0 = i + px * L;
-        X.oa0 = X.ob0;
-        X.sb = px * h;
-        X.sa = px * (h + 1);
-    }
+    X.ob = slab_index(a, h, L, i, j, k, px, ny);
+    X.oa = slab_index(a, h + 1, L, i, j, k, px, ny);
+    X.ls = a == 0 ? 1 : (a == 1 ? px : px * ny);
     return X;
 }
 
-template <typename T>
-__device__ __forceinline__ void adj_axis_vals(const AdjAxis<T>& X, int64_t k, AdjVals<T>& V) {
-    adj_vals(X.C, X.be_rd, X.al_rd, X.ob0 + k * X.sb, X.oa0 + k * X.sa, X.ls, V);
-}
-
-// The march axis: its slab layers are whole planes, layer L of a thread's
-// column at base + L * ms (ms: march stride), as its field nodes.
-template <typename T>
-__device__ __forceinline__ void adj_march_vals(const AdjointArgs<T>& A, const AdjCoef<T>& C,
-                                               int a, int shot, int64_t base, int64_t ms,
-                                               AdjVals<T>& V) {
-    const int s = C.side;
-    const T* be = (s ? A.side[a][1].be_rd : A.side[a][0].be_rd) + shot * A.xi_stride[a];
-    const T* al = (s ? A.side[a][1].al_rd : A.side[a][0].al_rd) + shot * A.psi_stride[a];
-    const int64_t o = base + (int64_t)(C.m - 1) * ms;
-    adj_vals(C, be, al, o, o, ms, V);
-}
-
-// One node's adjoint update.  (i, j, k) interior (2D: j is the march index
-// and k = 0), lin its element offset in the shot's field, km its march
-// index; fc, lc: fact and lam^{s+1} at the node; um/up: f lam^{s+1} one
-// node below/above along each axis; l2, qv, gv: lam^{s+2}, q^s and the
-// gradient at the node; X, Y, Z: CPML data of the x, y (3D) and march
-// axes with their slab values xv, yv, zv.
-template <typename T, int NDIM>
-__device__ __forceinline__ void adjoint_core(const AdjointArgs<T>& A, const AdjAxis<T>& X,
-                                             const AdjAxis<T>& Y, const AdjCoef<T>& Z,
-                                             const AdjVals<T>& xv, const AdjVals<T>& yv,
-                                             const AdjVals<T>& zv, int64_t km, int shot,
-                                             int64_t base, int64_t ms, int64_t lin, int64_t fs,
-                                             T fc, T lc, const T* um, const T* up, T l2, T qv,
-                                             T gv) {
-    const T uc = fc * lc;
-    T acc = T(2) * lc - l2;
-#pragma unroll
-    for (int a = 0; a < NDIM; ++a) acc += (up[a] - T(2) * uc + um[a]) * A.inv_d2[a];
-    const int M = NDIM - 1;
-    const int64_t xo = X.oa0 + km * X.sa;
-    if (X.C.ext) adj_term(X.C, xv, um[0], uc, up[0], A.inv_d[0], A.inv_d2[0], X.al_wr, xo, X.ls, acc);
-    int64_t yo = 0;
-    if (NDIM == 3) {
-        yo = Y.oa0 + km * Y.sa;
-        if (Y.C.ext)
-            adj_term(Y.C, yv, um[1], uc, up[1], A.inv_d[1], A.inv_d2[1], Y.al_wr, yo, Y.ls, acc);
-    }
-    const int64_t zo = base + (int64_t)(Z.m - 1) * ms;
-    T* zal = nullptr;
-    T* zbe = nullptr;
-    if (Z.ext) {
-        const int s = Z.side;
-        zal = (s ? A.side[M][1].al_wr : A.side[M][0].al_wr) + shot * A.psi_stride[M];
-        zbe = (s ? A.side[M][1].be_wr : A.side[M][0].be_wr) + shot * A.xi_stride[M];
-        adj_term(Z, zv, um[M], uc, up[M], A.inv_d[M], A.inv_d2[M], zal, zo, ms, acc);
-    }
-    A.lam0[fs + lin] = acc;
-    // beta^s = bK beta^{s+1} + f lam^s on every slab holding the node
-    if (X.C.ext && X.C.v1) X.be_wr[X.ob0 + km * X.sb] = X.C.bK1 * xv.b1 + fc * acc;
-    if (NDIM == 3 && Y.C.ext && Y.C.v1) Y.be_wr[Y.ob0 + km * Y.sb] = Y.C.bK1 * yv.b1 + fc * acc;
-    if (Z.ext && Z.v1) zbe[zo] = Z.bK1 * zv.b1 + fc * acc;
-    A.grad[fs + lin] = gv + acc * qv;
-}
-
-// Loads of one node that are not carried along the march: the in-plane
-// neighbours of lam^{s+1} and fact, lam^{s+2}, q^s and the gradient.
-template <typename T>
-struct AdjNear {
-    T lxm, lxp, fxm, fxp, lym, lyp, fym, fyp, l2, qv, gv;
+// Everything one shot reads at one node: lam^{s+1} at the node and its
+// axis neighbours, lam^{s+2}, q^s and (CP) the slab values of every axis
+// whose CPML range holds the node.
+template <typename T, int NDIM, bool CP>
+struct AdjShot {
+    T lc, lm[NDIM], lp[NDIM], l2, q;
+    AdjVals<T> v[CP ? NDIM : 1];
 };
 
-template <typename T, int NDIM>
-__device__ __forceinline__ void adj_load_near(const T* l1, const T* f, const T* l2, const T* q,
-                                              const T* g, int64_t lin, int64_t sy, AdjNear<T>& n) {
-    n.lxm = l1[lin - 1];
-    n.lxp = l1[lin + 1];
-    n.fxm = f[lin - 1];
-    n.fxp = f[lin + 1];
-    if (NDIM == 3) {
-        n.lym = l1[lin - sy];
-        n.lyp = l1[lin + sy];
-        n.fym = f[lin - sy];
-        n.fyp = f[lin + sy];
+template <typename T, int NDIM, bool CP>
+__device__ __forceinline__ void adj_load_shot(const AdjointArgs<T>& A, const AdjAxis<T>* X,
+                                              int g, int64_t lin, const int64_t* so,
+                                              AdjShot<T, NDIM, CP>& S) {
+    const int64_t fs = (int64_t)g * A.field_stride;
+    const T* l1 = A.lam1 + fs;
+    S.lc = l1[lin];
+#pragma unroll
+    for (int a = 0; a < NDIM; ++a) {
+        S.lm[a] = l1[lin - so[a]];
+        S.lp[a] = l1[lin + so[a]];
     }
-    n.l2 = l2[lin];
-    n.qv = q[lin];
-    n.gv = g[lin];
+    S.l2 = A.lam2[fs + lin];
+    S.q = A.q[fs + lin];
+    if constexpr (CP) {
+#pragma unroll
+        for (int a = 0; a < NDIM; ++a) {
+            if (!X[a].C.ext) continue;
+            const bool hi = X[a].C.side != 0;
+            const T* be = hi ? A.side[a][1].be_rd : A.side[a][0].be_rd;
+            const T* al = hi ? A.side[a][1].al_rd : A.side[a][0].al_rd;
+            adj_vals(X[a].C, be + g * A.xi_stride[a], al + g * A.psi_stride[a], X[a].ob,
+                     X[a].oa, X[a].ls, S.v[a]);
+        }
+    }
 }
 
-// One reverse step.  Block (64, BY) as the forward kernel: lane -> x,
-// threadIdx.y -> y row (3D) or march chunk (2D); each thread marches a
-// chunk of march_len nodes along the last axis.  f lam^{s+1} along the
-// march is carried in registers (one new plane per node), and every load
-// of node k+1, its CPML slab values included, is issued before node k is
-// computed.  Logical blocks: x tile fastest, then row/chunk group, chunk
-// (3D), shot, renumbered XCD-aware.
-template <typename T, int NDIM>
+// Shot g's reverse step at the node from its loads S: stores lam^s (and,
+// CP, the slab values) and returns lam^s q^s, the node's gradient term.
+template <typename T, int NDIM, bool CP>
+__device__ __forceinline__ T adj_shot(const AdjointArgs<T>& A, const AdjAxis<T>* X, int g,
+                                      int64_t lin, T fc, const T* fm, const T* fp,
+                                      const AdjShot<T, NDIM, CP>& S) {
+    const int64_t fs = (int64_t)g * A.field_stride;
+    const T uc = fc * S.lc;
+    T um[NDIM], up[NDIM];
+    T acc = T(2) * S.lc - S.l2;
+#pragma unroll
+    for (int a = 0; a < NDIM; ++a) {
+        um[a] = fm[a] * S.lm[a];
+        up[a] = fp[a] * S.lp[a];
+        acc += (up[a] - T(2) * uc + um[a]) * A.inv_d2[a];
+    }
+    if constexpr (CP) {
+#pragma unroll
+        for (int a = 0; a < NDIM; ++a) {
+            if (!X[a].C.ext) continue;
+            T* alw = (X[a].C.side ? A.side[a][1].al_wr : A.side[a][0].al_wr) +
+                     g * A.psi_stride[a];
+            adj_term(X[a].C, S.v[a], um[a], uc, up[a], A.inv_d[a], A.inv_d2[a], alw, X[a].oa,
+                     X[a].ls, acc);
+        }
+    }
+    A.lam0[fs + lin] = acc;
+    if constexpr (CP) {
+        // beta^s = bK beta^{s+1} + f lam^s on every slab holding the node
+#pragma unroll
+        for (int a = 0; a < NDIM; ++a) {
+            if (!(X[a].C.ext && X[a].C.v1)) continue;
+            T* bew = (X[a].C.side ? A.side[a][1].be_wr : A.side[a][0].be_wr) +
+                     g * A.xi_stride[a];
+            bew[X[a].ob] = X[a].C.bK1 * S.v[a].b1 + fc * acc;
+        }
+    }
+    return acc * S.q;
+}
+
+// One shot of the three-slot pipeline: refill slot F with shot g+2's
+// loads, then compute shot g from slot C (loaded two shots earlier).
+template <typename T, int NDIM, bool CP>
+__device__ __forceinline__ void adj_pipe(const AdjointArgs<T>& A, const AdjAxis<T>* X, int g,
+                                         int g1, int64_t lin, const int64_t* so, T fc,
+                                         const T* fm, const T* fp, AdjShot<T, NDIM, CP>& C,
+                                         AdjShot<T, NDIM, CP>& F, T& gsum) {
+    if (g >= g1) return;
+    if (g + 2 < g1) adj_load_shot<T, NDIM, CP>(A, X, g + 2, lin, so, F);
+    // keep the later shots' loads ahead of this shot's arithmetic
+    __builtin_amdgcn_sched_barrier(0);
+    gsum += adj_shot<T, NDIM, CP>(A, X, g, lin, fc, fm, fp, C);
+}
+
+// Tiles of axis a in block class c, and the u-th of them.
+__device__ __forceinline__ int adj_class_tiles(int c, int n, int lo, int hi, int a) {
+    return ((c >> a) & 1) ? n - (hi - lo) : hi - lo;
+}
+__device__ __forceinline__ int adj_class_tile(int c, int lo, int hi, int a, int u) {
+    if (!((c >> a) & 1)) return lo + u;
+    return u < lo ? u : hi + (u - lo);
+}
+
+// One reverse step.  Block (64, BY): lane -> x, threadIdx.y -> y row; a
+// thread owns one interior node (plane k in 3D) and walks the shots of
+// its shot group, so fact, the CPML coefficients and the gradient
+// accumulator of the node are read once per group and the group's
+// gradient terms are summed in registers (one read-modify-write of the
+// group's accumulator per step).  Shots run through a three-slot pipeline:
+// the loads of shots g+1 and g+2 are in flight while shot g is computed.
+// CP = false: the launch covers block class 0 (no node of the block in a
+// CPML range) and carries no slab code, so it runs with the registers of
+// the plain stencil; CP = true: classes 1 and up.  Logical blocks within a
+// class: x tile fastest, then row group, plane (3D), shot group,
+// renumbered XCD-aware.
+template <typename T, int NDIM, bool CP>
 __global__ __launch_bounds__(256) void adjoint_step(const AdjointArgs<T> A) {
     const int lb = logical_block(blockIdx.x, A.grid_blocks);
     if (lb >= A.logical_blocks) return;
-    const int64_t nx = A.n[0], ny = A.n[1], nz = A.n[2], px = A.px;
-    int rest = lb;
-    const int tx = rest % A.tiles_x;
-    rest /= A.tiles_x;
-    const int ty = rest % A.tiles_y;
-    rest /= A.tiles_y;
+    const int64_t nx = A.n[0], ny = A.n[1], px = A.px;
+    int cls = 0;
+    if (CP)
+        for (int c = 1; c < 8; ++c)
+            if (lb >= A.cls_start[c]) cls = c;
+    int rest = lb - A.cls_start[cls];
+    int t[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int na = adj_class_tiles(cls, A.ntiles[a], A.int_lo[a], A.int_hi[a], a);
+        t[a] = adj_class_tile(cls, A.int_lo[a], A.int_hi[a], a, rest % na);
+        rest /= na;
+    }
+    const int grp = rest;
+    const int64_t k = NDIM == 3 ? 1 + t[2] : 0;
+    const int64_t i = (int64_t)t[0] * kTileX + threadIdx.x;
+    // one row per wavefront: j (and the y axis's CPML data) is wave-uniform
     const int wy = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
-    int chunk, shot;
-    int64_t j = 0;
-    if (NDIM == 3) {
-        chunk = rest % A.nchunks;
-        shot = rest / A.nchunks;
-        j = 1 + (int64_t)ty * blockDim.y + wy;
-        if (j > ny - 2) return;
-    } else {
-        shot = rest;
-        chunk = ty * (int)blockDim.y + wy;
-        if (chunk >= A.nchunks) return;
-    }
-    const int64_t i = (int64_t)tx * kTileX + threadIdx.x;
-    if (i < 1 || i > nx - 2) return;
-    const int64_t mext = NDIM == 3 ? nz : ny;
-    const int64_t k0 = 1 + (int64_t)chunk * A.march_len;
-    const int64_t k1 = k0 + A.march_len < mext - 1 ? k0 + A.march_len : mext - 1;
-    if (k0 >= k1) return;
-    const int M = NDIM - 1;
-    const int64_t ms = NDIM == 3 ? px * ny : px;  // march stride
-    const int64_t fs = (int64_t)shot * A.field_stride;
-    const T* l1 = A.lam1 + fs;
-    const T* l2 = A.lam2 + fs;
-    const T* qq = A.q + fs;
-    const T* gg = A.grad + fs;
+    const int64_t j = 1 + (int64_t)t[1] * blockDim.y + wy;
+    if (i < 1 || i > nx - 2 || j > ny - 2) return;
+    const int64_t so[3] = {1, px, px * ny};
+    const int64_t lin = i + px * j + so[2] * k;
     const T* f = A.fact;
-    const int64_t base = NDIM == 3 ? i + px * j : i;
-    int64_t lin = base + k0 * ms;
-    T u_m = f[lin - ms] * l1[lin - ms];
-    T lc = l1[lin], fc = f[lin];
-    T lp = l1[lin + ms], fp = f[lin + ms];
-    AdjNear<T> nr;
-    adj_load_near<T, NDIM>(l1, f, l2, qq, gg, lin, px, nr);
-    const AdjAxis<T> X = adj_axis<T, NDIM>(A, 0, shot, i, j);
-    AdjAxis<T> Y;
-    if (NDIM == 3) Y = adj_axis<T, NDIM>(A, 1, shot, i, j);
-    else Y.C.ext = false;
-    AdjCoef<T> Z = adj_coef<T>(A, M, k0);
-    AdjVals<T> xv = {}, yv = {}, zv = {};
-    if (X.C.ext) adj_axis_vals(X, k0, xv);
-    if (NDIM == 3 && Y.C.ext) adj_axis_vals(Y, k0, yv);
-    if (Z.ext) adj_march_vals(A, Z, M, shot, base, ms, zv);
-    for (int64_t k = k0; k < k1; ++k, lin += ms) {
-        // next node's loads (planes stay inside the field at the chunk end)
-        const int64_t kk = k + 2 < mext ? k + 2 : mext - 1;
-        const T lpp = l1[base + kk * ms], fpp = f[base + kk * ms];
-        AdjNear<T> nn;
-        adj_load_near<T, NDIM>(l1, f, l2, qq, gg, lin + ms, px, nn);
-        const AdjCoef<T> Zn = adj_coef<T>(A, M, k + 1);
-        AdjVals<T> xn = {}, yn = {}, zn = {};
-        if (X.C.ext) adj_axis_vals(X, k + 1, xn);
-        if (NDIM == 3 && Y.C.ext) adj_axis_vals(Y, k + 1, yn);
-        if (Zn.ext) adj_march_vals(A, Zn, M, shot, base, ms, zn);
-        // keep the next node's loads ahead of this node's arithmetic
-        __builtin_amdgcn_sched_barrier(0);
-        T um[3], up[3];
-        um[0] = nr.fxm * nr.lxm;
-        up[0] = nr.fxp * nr.lxp;
-        if (NDIM == 3) {
-            um[1] = nr.fym * nr.lym;
-            up[1] = nr.fyp * nr.lyp;
-        }
-        um[M] = u_m;
-        up[M] = fp * lp;
-        adjoint_core<T, NDIM>(A, X, Y, Z, xv, yv, zv, k, shot, base, ms, lin, fs, fc, lc, um, up,
-                              nr.l2, nr.qv, nr.gv);
-        u_m = fc * lc;
-        lc = lp;
-        fc = fp;
-        lp = lpp;
-        fp = fpp;
-        nr = nn;
-        Z = Zn;
-        xv = xn;
-        yv = yn;
-        zv = zn;
+    const T fc = f[lin];
+    T fm[NDIM], fp[NDIM];
+#pragma unroll
+    for (int a = 0; a < NDIM; ++a) {
+        fm[a] = f[lin - so[a]];
+        fp[a] = f[lin + so[a]];
     }
+    AdjAxis<T> X[CP ? NDIM : 1];
+    if constexpr (CP) {
+#pragma unroll
+        for (int a = 0; a < NDIM; ++a) X[a] = adj_axis<T>(A, a, i, j, k);
+    }
+    const int g0 = grp * A.group;
+    const int g1 = g0 + A.group < A.nshots ? g0 + A.group : A.nshots;
+    T gsum = T(0);
+    AdjShot<T, NDIM, CP> s0, s1, s2;
+    adj_load_shot<T, NDIM, CP>(A, X, g0, lin, so, s0);
+    if (g0 + 1 < g1) adj_load_shot<T, NDIM, CP>(A, X, g0 + 1, lin, so, s1);
+    for (int g = g0; g < g1; g += 3) {
+        adj_pipe<T, NDIM, CP>(A, X, g, g1, lin, so, fc, fm, fp, s0, s2, gsum);
+        adj_pipe<T, NDIM, CP>(A, X, g + 1, g1, lin, so, fc, fm, fp, s1, s0, gsum);
+        adj_pipe<T, NDIM, CP>(A, X, g + 2, g1, lin, so, fc, fm, fp, s2, s1, gsum);
+    }
+    T* gr = A.grad + (int64_t)grp * A.field_stride;
+    gr[lin] += gsum;
 }
 
 // Trace cotangent injection of step it0 (0-based) after adjoint_step:
 // for each distinct interior receiver node of a shot, r = the sum of the
-// cotangents of its receivers; lam^s += r, fbar += r q^s, and beta^s +=
-// f r on every absorbing slab holding the node (beta^s = f lam^s + ...).
+// cotangents of its receivers; lam^s += r, beta^s += f r on every
+// absorbing slab holding the node (beta^s = f lam^s + ...), and r q^s,
+// the node's extra gradient term, is added to the node's own entry of rq
+// (entry u of the receiver node list), which fold_receivers adds to the
+// gradient.  One block per shot; entries of different shots never share
+// memory, so the shots run in parallel and the sums stay in a fixed order.
 template <typename T>
 struct AdjIoArgs {
     T* lam0;
-    T* grad;
+    T* rq;                     // per receiver node entry: sum of r q^s
     const T* q;
     const T* fact;
     int64_t field_stride;
@@ -452,7 +423,7 @@ __global__ __launch_bounds__(256) void adjoint_inject(const AdjIoArgs<T> A, int6
             r += A.tbar[(int64_t)A.rec_idx[e] * A.nt + it0];
         const int64_t lin = A.node_lin[u];
         A.lam0[fs + lin] += r;
-        A.grad[fs + lin] += r * A.q[fs + lin];
+        A.rq[u] += r * A.q[fs + lin];
         const T fr = A.fact[lin] * r;
         const int64_t ny = A.n[1];
         const int64_t c[3] = {lin % A.px, (lin / A.px) % ny, lin / (A.px * ny)};
@@ -468,17 +439,35 @@ __global__ __launch_bounds__(256) void adjoint_inject(const AdjIoArgs<T> A, int6
     }
 }
 
-// Fold the per-shot partial gradients of the last reverse steps into the
-// FP64 total: total += sum over shots (in shot order), partials = 0.  The
-// partial sums then run over at most one checkpoint segment in the plan's
-// precision, and the long sum over time and shots is FP64.
+// Add the receiver gradient terms rq to the gradient total and clear them:
+// for each distinct receiver node v over all shots, total[lin[v]] += the
+// sum of its entries rq[idx[e]], start[v] <= e < start[v+1], in shot
+// order.
+template <typename T, typename A>
+__global__ __launch_bounds__(256) void fold_receivers(T* rq, A* total, const int64_t* lin,
+                                                      const int* start, const int* idx, int count) {
+    const int v = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (v >= count) return;
+    A acc = A(0);
+    for (int e = start[v]; e < start[v + 1]; ++e) {
+        acc += (A)rq[idx[e]];
+        rq[idx[e]] = T(0);
+    }
+    total[lin[v]] += acc;
+}
+
+// Fold the shot groups' partial gradients of the last reverse steps into
+// the FP64 total: total += sum over groups (in group order), partials = 0.
+// The partial sums then run over at most one checkpoint segment and one
+// group of shots in the plan's precision, and the long sum over time and
+// groups is FP64.
 template <typename T>
 __global__ __launch_bounds__(256) void fold_gradient(T* grad, double* total, int64_t field_stride,
-                                                     int nshots) {
+                                                     int ngroups) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= field_stride) return;
     double acc = total[t];
-    for (int s = 0; s < nshots; ++s) {
+    for (int s = 0; s < ngroups; ++s) {
         T& g = grad[(int64_t)s * field_stride + t];
         acc += (double)g;
         g = T(0);
@@ -486,13 +475,13 @@ __global__ __launch_bounds__(256) void fold_gradient(T* grad, double* total, int
     total[t] = acc;
 }
 
-// Sum of the per-shot gradients into shot 0's accumulator.
+// Sum of the shot groups' gradients into group 0's accumulator.
 template <typename T>
-__global__ __launch_bounds__(256) void sum_shots(T* grad, int64_t field_stride, int nshots) {
+__global__ __launch_bounds__(256) void sum_shots(T* grad, int64_t field_stride, int ngroups) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= field_stride) return;
     T acc = grad[t];
-    for (int s = 1; s < nshots; ++s) acc += grad[(int64_t)s * field_stride + t];
+    for (int s = 1; s < ngroups; ++s) acc += grad[(int64_t)s * field_stride + t];
     grad[t] = acc;
 }
 

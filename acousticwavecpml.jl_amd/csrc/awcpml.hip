@@ -155,6 +155,7 @@ struct awcpml_plan {
     // before they are folded into the FP64 total (0: one checkpoint
     // segment; -1: never, the whole sum stays in the plan's precision)
     int64_t grad_fold = 0;
+    int adj_group = 0;      // shots per thread of the reverse step, 0: automatic
     void* q_cur = nullptr;  // d(pnew)/d(fact) output of the next stencil launch
 
     // two time steps per launch (3D): step t+2's field, psi and xi go to
@@ -1944,11 +1945,102 @@ int awcpml_get_tiling(const awcpml_plan* P, int* block_y, int* march_len, int* p
 // segment a forward replay that stores q^s = d(pnew)/d(fact) for every
 // step of the segment, followed by the reverse-time steps of the segment
 // (adjoint_kernels.h).  Memory: ck_count checkpoints of 3 fields + slabs,
-// ck_every q fields, 3 adjoint fields, 2 x slabs, one gradient per shot.
+// ck_every q fields, 3 adjoint fields, 2 x slabs, one gradient per shot
+// group.
 // ---------------------------------------------------------------------
 namespace {
 
 constexpr int kAdjBlockY = 4;  // wavefronts per block of the reverse step
+// Blocks one shot group of the reverse step should at least fill: 8
+// blocks of 4 wavefronts per CU on 256 CUs.
+constexpr int64_t kAdjTargetBlocks = 2048;
+
+// Launch shape of the reverse step: blocks per shot group (x tiles x row
+// groups x interior planes) and the shots per thread.  Fewer groups mean
+// fewer gradient accumulators and fewer passes over fact; the automatic
+// choice splits the shots only as far as the grid needs to fill the chip.
+struct AdjShape {
+    int tiles_x, tiles_y;
+    int64_t blocks_per_group;
+    int group, ngroups;
+    // blocks with CPML nodes (classes 1..7) walk fewer shots per thread:
+    // they are few and carry the slab loads, so the launch needs more of
+    // them in flight; the gradient has ngroups_cp >= ngroups accumulators
+    int group_cp, ngroups_cp;
+    // block classes (bit a: the tiles of axis a holding CPML nodes); per
+    // axis the tiles and the range of those without CPML nodes
+    int ntiles[3], int_lo[3], int_hi[3];
+    int64_t class_blocks[8];  // all shot groups
+};
+
+// Interior tile range [*lo, *hi) of one axis from per-tile flags (set: the
+// tile holds CPML nodes).  The flagged tiles must lie before and after the
+// range; otherwise the whole axis counts as flagged (lo = hi = 0), which
+// runs every tile with the CPML terms.
+void interior_range(const std::vector<char>& slab, int* lo, int* hi) {
+    const int n = (int)slab.size();
+    int a = 0;
+    while (a < n && slab[a]) ++a;
+    int b = n;
+    while (b > a && slab[b - 1]) --b;
+    for (int t = a; t < b; ++t)
+        if (slab[t]) {
+            *lo = *hi = 0;
+            return;
+        }
+    *lo = a;
+    *hi = b;
+}
+
+// Whether node c of axis a lies in the range 0 <= m <= halo+1 of an
+// absorbing side, where the reverse step has slab terms (adj_coef).
+bool adj_node_ext(const awcpml_plan* P, int a, int64_t c) {
+    const int64_t h = P->halo, n = P->n[a];
+    return (P->active[a][0] && c <= h + 1) || (P->active[a][1] && c >= n - h - 2);
+}
+
+AdjShape adj_shape(const awcpml_plan* P) {
+    AdjShape S;
+    S.tiles_x = (int)(round_up(P->n[0], awcpml::kTileX) / awcpml::kTileX);
+    S.tiles_y = (int)((P->n[1] - 2 + kAdjBlockY - 1) / kAdjBlockY);
+    S.blocks_per_group = (int64_t)S.tiles_x * S.tiles_y * (P->ndim == 3 ? P->n[2] - 2 : 1);
+    int g = P->adj_group;
+    if (g <= 0) {
+        const int64_t want = (kAdjTargetBlocks + S.blocks_per_group - 1) / S.blocks_per_group;
+        const int ng = (int)std::max<int64_t>(1, std::min<int64_t>(P->nshots, want));
+        g = (P->nshots + ng - 1) / ng;
+    }
+    S.group = std::min(g, P->nshots);
+    S.ngroups = (P->nshots + S.group - 1) / S.group;
+    S.group_cp = P->adj_group > 0 ? S.group : std::max(1, (S.group + 3) / 4);
+    S.ngroups_cp = (P->nshots + S.group_cp - 1) / S.group_cp;
+    // tiles: x tiles of kTileX lanes (nodes 1..nx-2), row groups of
+    // kAdjBlockY rows from row 1, planes 1..nz-2 (3D)
+    S.ntiles[0] = S.tiles_x;
+    S.ntiles[1] = S.tiles_y;
+    S.ntiles[2] = P->ndim == 3 ? (int)(P->n[2] - 2) : 1;
+    for (int a = 0; a < 3; ++a) {
+        std::vector<char> flag(S.ntiles[a], 0);
+        if (a < P->ndim)
+            for (int t = 0; t < S.ntiles[a]; ++t) {
+                const int64_t c0 = a == 0 ? (int64_t)t * awcpml::kTileX
+                                          : (a == 1 ? 1 + (int64_t)t * kAdjBlockY : 1 + t);
+                const int64_t c1 = a == 0 ? c0 + awcpml::kTileX : (a == 1 ? c0 + kAdjBlockY : c0 + 1);
+                for (int64_t c = std::max<int64_t>(c0, 1); c < c1 && c <= P->n[a] - 2 && !flag[t]; ++c)
+                    flag[t] = adj_node_ext(P, a, c);
+            }
+        interior_range(flag, &S.int_lo[a], &S.int_hi[a]);
+    }
+    for (int c = 0; c < 8; ++c) {
+        int64_t nb = c ? S.ngroups_cp : S.ngroups;
+        for (int a = 0; a < 3; ++a) {
+            const int in = S.int_hi[a] - S.int_lo[a];
+            nb *= ((c >> a) & 1) ? S.ntiles[a] - in : in;
+        }
+        S.class_blocks[c] = nb;
+    }
+    return S;
+}
 
 struct CkLayout {
     size_t field;            // bytes of one field buffer (all shots)
@@ -2022,6 +2114,10 @@ struct AdjBuffers {
     void* node_start = nullptr;
     void* rec_start = nullptr;
     void* rec_idx = nullptr;
+    void* rq = nullptr;        // per receiver node entry: its gradient terms
+    void* rf_lin = nullptr;    // distinct receiver nodes over all shots
+    void* rf_start = nullptr;  // CSR offsets into rf_idx
+    void* rf_idx = nullptr;    // entries of each distinct node, in shot order
     ~AdjBuffers() {
         for (auto& v : lam) dfree(v);
         for (int a = 0; a < 3; ++a)
@@ -2038,12 +2134,18 @@ struct AdjBuffers {
         dfree(node_start);
         dfree(rec_start);
         dfree(rec_idx);
+        dfree(rq);
+        dfree(rf_lin);
+        dfree(rf_start);
+        dfree(rf_idx);
     }
 };
 
+// cp = false: the launch over block class 0 (no CPML nodes); true: over
+// classes 1..7.
 template <typename T>
 awcpml::AdjointArgs<T> make_adj_args(const awcpml_plan* P, const AdjBuffers& Bf, int lr, int par,
-                                     const void* q) {
+                                     const void* q, bool cp) {
     awcpml::AdjointArgs<T> A;
     memset(&A, 0, sizeof A);
     for (int a = 0; a < 3; ++a) A.n[a] = P->n[a];
@@ -2077,17 +2179,22 @@ awcpml::AdjointArgs<T> make_adj_args(const awcpml_plan* P, const AdjBuffers& Bf,
     }
     A.ndim = P->ndim;
     A.nshots = P->nshots;
-    A.march_len = 32;
-    const int64_t mext = P->ndim == 3 ? P->n[2] : P->n[1];
-    A.nchunks = (int)((mext - 2 + A.march_len - 1) / A.march_len);
-    A.tiles_x = (int)(round_up(P->n[0], awcpml::kTileX) / awcpml::kTileX);
-    if (P->ndim == 3) {
-        A.tiles_y = (int)((P->n[1] - 2 + kAdjBlockY - 1) / kAdjBlockY);
-        A.logical_blocks = A.tiles_x * A.tiles_y * A.nchunks * P->nshots;
-    } else {
-        A.tiles_y = (A.nchunks + kAdjBlockY - 1) / kAdjBlockY;
-        A.logical_blocks = A.tiles_x * A.tiles_y * P->nshots;
+    const AdjShape S = adj_shape(P);
+    A.group = cp ? S.group_cp : S.group;
+    A.tiles_x = S.tiles_x;
+    A.tiles_y = S.tiles_y;
+    for (int a = 0; a < 3; ++a) {
+        A.ntiles[a] = S.ntiles[a];
+        A.int_lo[a] = S.int_lo[a];
+        A.int_hi[a] = S.int_hi[a];
     }
+    int64_t total = 0;
+    A.cls_start[0] = 0;
+    for (int c = 0; c < 8; ++c) {
+        if (cp == (c > 0)) total += S.class_blocks[c];
+        A.cls_start[c + 1] = cp ? (int)total : 0;
+    }
+    A.logical_blocks = (int)total;
     A.grid_blocks = (int)round_up(A.logical_blocks, awcpml::kNumXcd);
     return A;
 }
@@ -2113,8 +2220,12 @@ int adjoint_t(awcpml_plan* P, const double* tbar, double* fbar) {
                 if (xb) HIP_TRY(hipMemsetAsync(Bf.be[a][s][b], 0, xb, P->stream));
             }
     if ((st = dalloc(&Bf.q, fb * (size_t)K))) return st;
-    if ((st = dalloc(&Bf.grad, fb))) return st;
-    HIP_TRY(hipMemsetAsync(Bf.grad, 0, fb, P->stream));
+    const AdjShape shape = adj_shape(P);
+    if ((uint64_t)shape.blocks_per_group * shape.ngroups_cp > (uint64_t)INT32_MAX)
+        return fail(AWCPML_ERR_ARG, "grid too large for the reverse step");
+    const size_t gb = (size_t)P->field_elems * shape.ngroups_cp * sizeof(T);
+    if ((st = dalloc(&Bf.grad, gb))) return st;
+    HIP_TRY(hipMemsetAsync(Bf.grad, 0, gb, P->stream));
     const bool fold = P->grad_fold >= 0;
     const int64_t fold_every = P->grad_fold > 0 ? P->grad_fold : K;
     const int64_t nb_fold = (P->field_elems + 255) / 256;
@@ -2122,11 +2233,20 @@ int adjoint_t(awcpml_plan* P, const double* tbar, double* fbar) {
         if ((st = dalloc(&Bf.grad64, (size_t)P->field_elems * 8))) return st;
         HIP_TRY(hipMemsetAsync(Bf.grad64, 0, (size_t)P->field_elems * 8, P->stream));
     }
-    auto fold_now = [&]() -> int {
-        awcpml::fold_gradient<T><<<dim3((unsigned)nb_fold), dim3(256), 0, P->stream>>>(
-            (T*)Bf.grad, (double*)Bf.grad64, P->field_elems, P->nshots);
+    int rf_count = 0;  // distinct receiver nodes (set below)
+    auto fold_rq = [&](auto* total) -> int {
+        if (rf_count == 0) return AWCPML_OK;
+        awcpml::fold_receivers<<<dim3((unsigned)((rf_count + 255) / 256)), dim3(256), 0, P->stream>>>(
+            (T*)Bf.rq, total, (const int64_t*)Bf.rf_lin, (const int*)Bf.rf_start,
+            (const int*)Bf.rf_idx, rf_count);
         HIP_TRY(hipGetLastError());
         return AWCPML_OK;
+    };
+    auto fold_now = [&]() -> int {
+        awcpml::fold_gradient<T><<<dim3((unsigned)nb_fold), dim3(256), 0, P->stream>>>(
+            (T*)Bf.grad, (double*)Bf.grad64, P->field_elems, shape.ngroups_cp);
+        HIP_TRY(hipGetLastError());
+        return fold_rq((double*)Bf.grad64);
     };
 
     // trace cotangents and the distinct interior receiver nodes per shot
@@ -2171,6 +2291,35 @@ int adjoint_t(awcpml_plan* P, const double* tbar, double* fbar) {
     }
     HIP_TRY(hipMemcpyAsync(Bf.node_start, node_start.data(), node_start.size() * 4, hipMemcpyHostToDevice, P->stream));
     HIP_TRY(hipMemcpyAsync(Bf.rec_start, rec_start.data(), rec_start.size() * 4, hipMemcpyHostToDevice, P->stream));
+    // the receiver gradient terms: one accumulator per entry of node_lin,
+    // folded per distinct node over all shots in shot order
+    std::vector<int64_t> rf_lin;
+    std::vector<int> rf_start(1, 0), rf_idx;
+    {
+        std::vector<std::pair<int64_t, int>> v;
+        for (size_t u = 0; u < node_lin.size(); ++u) v.push_back({node_lin[u], (int)u});
+        std::stable_sort(v.begin(), v.end(),
+                         [](const std::pair<int64_t, int>& x, const std::pair<int64_t, int>& y) {
+                             return x.first < y.first;
+                         });
+        for (size_t e = 0; e < v.size(); ++e) {
+            if (e && v[e].first != v[e - 1].first) rf_start.push_back((int)rf_idx.size());
+            if (e == 0 || v[e].first != v[e - 1].first) rf_lin.push_back(v[e].first);
+            rf_idx.push_back(v[e].second);
+        }
+        if (!v.empty()) rf_start.push_back((int)rf_idx.size());
+    }
+    rf_count = (int)rf_lin.size();
+    if (rf_count) {
+        if ((st = dalloc(&Bf.rq, node_lin.size() * sizeof(T)))) return st;
+        HIP_TRY(hipMemsetAsync(Bf.rq, 0, node_lin.size() * sizeof(T), P->stream));
+        if ((st = dalloc(&Bf.rf_lin, rf_lin.size() * 8))) return st;
+        if ((st = dalloc(&Bf.rf_start, rf_start.size() * 4))) return st;
+        if ((st = dalloc(&Bf.rf_idx, rf_idx.size() * 4))) return st;
+        HIP_TRY(hipMemcpyAsync(Bf.rf_lin, rf_lin.data(), rf_lin.size() * 8, hipMemcpyHostToDevice, P->stream));
+        HIP_TRY(hipMemcpyAsync(Bf.rf_start, rf_start.data(), rf_start.size() * 4, hipMemcpyHostToDevice, P->stream));
+        HIP_TRY(hipMemcpyAsync(Bf.rf_idx, rf_idx.data(), rf_idx.size() * 4, hipMemcpyHostToDevice, P->stream));
+    }
 
     awcpml::AdjIoArgs<T> io;
     memset(&io, 0, sizeof io);
@@ -2186,7 +2335,7 @@ int adjoint_t(awcpml_plan* P, const double* tbar, double* fbar) {
     io.px = P->px;
     io.halo = P->halo;
     io.ndim = P->ndim;
-    io.grad = (T*)Bf.grad;
+    io.rq = (T*)Bf.rq;
 
     int lr = 0, par = 0;
     for (int64_t seg = P->ck_count - 1; seg >= 0; --seg) {
@@ -2201,13 +2350,20 @@ int adjoint_t(awcpml_plan* P, const double* tbar, double* fbar) {
         }
         for (int64_t s = c1; s > c0; --s) {
             const void* q = (const char*)Bf.q + (size_t)(s - c0 - 1) * fb;
-            awcpml::AdjointArgs<T> A = make_adj_args<T>(P, Bf, lr, par, q);
-            const dim3 grid(A.grid_blocks), block(awcpml::kTileX, kAdjBlockY);
-            if (P->ndim == 3)
-                awcpml::adjoint_step<T, 3><<<grid, block, 0, P->stream>>>(A);
-            else
-                awcpml::adjoint_step<T, 2><<<grid, block, 0, P->stream>>>(A);
-            HIP_TRY(hipGetLastError());
+            awcpml::AdjointArgs<T> A;
+            for (int cp = 0; cp < 2; ++cp) {  // blocks without, then with CPML nodes
+                A = make_adj_args<T>(P, Bf, lr, par, q, cp != 0);
+                if (A.logical_blocks == 0) continue;
+                const dim3 grid(A.grid_blocks), block(awcpml::kTileX, kAdjBlockY);
+                if (P->ndim == 3) {
+                    if (cp) awcpml::adjoint_step<T, 3, true><<<grid, block, 0, P->stream>>>(A);
+                    else awcpml::adjoint_step<T, 3, false><<<grid, block, 0, P->stream>>>(A);
+                } else {
+                    if (cp) awcpml::adjoint_step<T, 2, true><<<grid, block, 0, P->stream>>>(A);
+                    else awcpml::adjoint_step<T, 2, false><<<grid, block, 0, P->stream>>>(A);
+                }
+                HIP_TRY(hipGetLastError());
+            }
             if (!node_lin.empty()) {
                 io.lam0 = A.lam0;
                 io.q = (const T*)q;
@@ -2234,12 +2390,13 @@ int adjoint_t(awcpml_plan* P, const double* tbar, double* fbar) {
             for (int64_t i = 0; i < nx; ++i) fbar[r * nx + i] = host[(size_t)(r * P->px + i)];
         return AWCPML_OK;
     }
-    if (P->nshots > 1) {
+    if (shape.ngroups_cp > 1) {
         const int64_t nb = (P->field_elems + 255) / 256;
         awcpml::sum_shots<T><<<dim3((unsigned)nb), dim3(256), 0, P->stream>>>((T*)Bf.grad, P->field_elems,
-                                                                            P->nshots);
+                                                                            shape.ngroups_cp);
         HIP_TRY(hipGetLastError());
     }
+    if ((st = fold_rq((T*)Bf.grad))) return st;
     std::vector<T> host((size_t)P->field_elems);
     HIP_TRY(hipMemcpyAsync(host.data(), Bf.grad, host.size() * sizeof(T), hipMemcpyDeviceToHost, P->stream));
     HIP_TRY(hipStreamSynchronize(P->stream));
@@ -2296,6 +2453,25 @@ int awcpml_set_gradient_fold(awcpml_plan* P, int64_t steps) {
     if (st) return st;
     if (steps < -1) return fail(AWCPML_ERR_ARG, "steps must be >= -1 (got %lld)", (long long)steps);
     P->grad_fold = steps;
+    return AWCPML_OK;
+}
+
+int awcpml_set_adjoint_group(awcpml_plan* P, int shots) {
+    g_err.clear();
+    int st = check_plan(P);
+    if (st) return st;
+    if (shots < 0 || shots > P->nshots)
+        return fail(AWCPML_ERR_ARG, "shots per thread must be in 0..%d (got %d)", P->nshots, shots);
+    P->adj_group = shots;
+    return AWCPML_OK;
+}
+
+int awcpml_get_adjoint_group(const awcpml_plan* P, int* shots) {
+    g_err.clear();
+    int st = check_plan(P);
+    if (st) return st;
+    if (!shots) return fail(AWCPML_ERR_ARG, "NULL argument");
+    *shots = adj_shape(P).group;
     return AWCPML_OK;
 }
 

@@ -329,14 +329,27 @@ int awcpml_run_noalloc(int ndim, int64_t nx, int64_t ny, int64_t nz, int halo, i
                        const int64_t *recpos, void *traces, void *pend, void *stream);
 
 /* Precision of the gradient sum of awcpml_adjoint.  Each reverse step adds
- * lam * q to a per-shot partial sum in the plan's precision; every `steps`
- * reverse steps (and at the end of every checkpoint segment) the partial
- * sums of all shots are added, in shot order, to an FP64 total and reset.
+ * lam * q of every shot of a shot group (awcpml_set_adjoint_group) to the
+ * group's partial sum in the plan's precision; every `steps` reverse steps
+ * (and at the end of every checkpoint segment) the partial sums of all
+ * groups are added, in group order, to an FP64 total and reset.
  * steps = 0 (the default): once per checkpoint segment.  steps = -1: never;
  * the whole sum over time and shots stays in the plan's precision (the
  * cheapest, and for FP32 plans on long runs the least accurate).  Extension
  * beyond the reference, which has no gradient (README.md:297). */
 int awcpml_set_gradient_fold(awcpml_plan *plan, int64_t steps);
+
+/* Shots per thread of awcpml_adjoint's reverse step: a thread walks
+ * `shots` consecutive shots at its node, so fact and the node's gradient
+ * accumulator are read once per group and the group's contributions are
+ * summed in registers.  0 (the default): the fewest groups whose launch
+ * still fills the GPU, and a quarter of that group size for the blocks
+ * holding CPML nodes, which are few.  awcpml_get_adjoint_group reports the
+ * value in use (for the blocks without CPML nodes).
+ * A tuning knob of this library's reverse step; the reference has no
+ * gradient (README.md:297). */
+int awcpml_set_adjoint_group(awcpml_plan *plan, int shots);
+int awcpml_get_adjoint_group(const awcpml_plan *plan, int *shots);
 
 /* Message of the calling thread's last failed call ("" if none). */
 const char *awcpml_last_error(void);

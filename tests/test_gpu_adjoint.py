@@ -34,6 +34,11 @@ CASES = {
     "2d_open": lambda: small2d(freetop=False, n=(41, 37), halo=5, nt=120),
     "3d_freetop": lambda: small3d(freetop=True, nt=60),
     "3d_open_2src": lambda: small3d(freetop=False, nt=50, nsrc=2),
+    # grids whose x tiles, row groups (and 3D planes) include some without
+    # CPML nodes, so both reverse-step launches (with and without slab
+    # terms) run
+    "2d_classes": lambda: small2d(freetop=False, n=(150, 70), halo=5, nt=90),
+    "3d_classes": lambda: small3d(freetop=False, n=(140, 30, 24), halo=4, nt=30),
 }
 
 
@@ -83,7 +88,16 @@ def test_multishot_gradient_is_sum_of_shots():
     M = plan_from_oracle(o, nshots=3, pos_list=pos)
     M.checkpoint_run()
     obs = [M.traces(s) * 0.3 for s in range(3)]
-    J, g = fwi.misfit_gradient(M, obs, every=5)
+    # shots per thread of the reverse step: 1, 2 (a partial last group) and
+    # 3 (one group); the shots share their receiver nodes
+    grads = []
+    for group in (1, 2, 3):
+        M.set_adjoint_group(group)
+        assert M.adjoint_group() == group
+        J, g = fwi.misfit_gradient(M, obs, every=5)
+        grads.append(g)
+    for g2 in grads[1:]:
+        assert rel_l2(g2, grads[0]) <= 1e-14
     total = np.zeros(o.n)
     Jsum = 0.0
     for s in range(3):
@@ -167,6 +181,9 @@ def test_adjoint_errors():
     P.set_model(o.fact(), [1.0, 1.0], [1.0, 1.0])  # invalidates the checkpoints
     with pytest.raises(_lib.AwcpmlError, match="checkpoint"):
         P.adjoint(np.zeros((o.nt, o.nrecs)))
+    for bad in (-1, 2):  # one shot: 0 (automatic) or 1
+        with pytest.raises(_lib.AwcpmlError, match="shots per thread"):
+            P.set_adjoint_group(bad)
     P.close()
     # 15 points < 2*6+4 along an absorbing x axis
     P2 = awcpml.Plan((15, 40), 6)

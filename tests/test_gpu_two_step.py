@@ -67,6 +67,9 @@ CASES = {
     "open_2src": dict(n=(37, 29, 23), halo=5, freetop=False, nt=57, nsrc=2),
     "xtiles": dict(n=(150, 33, 26), halo=6, freetop=True, nt=45),
     "ytiles": dict(n=(40, 71, 24), halo=4, freetop=False, nt=39),
+    # x tiles, row groups and march chunks both with and without CPML nodes:
+    # every block class of a launch is non-empty
+    "classes": dict(n=(197, 64, 61), halo=5, freetop=False, nt=31),
 }
 
 
