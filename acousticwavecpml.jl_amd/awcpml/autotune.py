@@ -47,7 +47,8 @@ def two_step_candidates(plan):
     n_march = plan.n[2] - 2
     out = []
     for w in ((16, 8) if plan.dtype != 0 else (8, 4)):
-        lens = (24, 48, 96, 144) if w == 16 else (24, 48, 96)
+        # FP64 on 298^3 (8 waves): chunk 40 measured 3 % faster than 48
+        lens = (24, 48, 96, 144) if w == 16 else (24, 32, 40, 48, 96)
         pfs = (1, 2, 3) if w == 16 else (1, 2)
         for ml in sorted({min(n_march, v) for v in lens}):
             out += [(w, ml, pf) for pf in pfs]

@@ -166,6 +166,16 @@ checkpoint_run!(plan::HIPPlan, every::Integer=0) =
     hipcheck(ccall((:awcpml_checkpoint_run, libawcpml), Cint, (Ptr{Cvoid}, Int64),
                    plan.handle, every))
 
+"Shots per thread of the reverse step (0: sized to the grid)."
+set_adjoint_group!(plan::HIPPlan, shots::Integer) =
+    hipcheck(ccall((:awcpml_set_adjoint_group, libawcpml), Cint, (Ptr{Cvoid}, Cint),
+                   plan.handle, shots))
+
+"Reverse steps summed in the plan's precision before the FP64 fold (0: a checkpoint segment, -1: never)."
+set_gradient_fold!(plan::HIPPlan, steps::Integer) =
+    hipcheck(ccall((:awcpml_set_gradient_fold, libawcpml), Cint, (Ptr{Cvoid}, Int64),
+                   plan.handle, steps))
+
 "dJ/dfact summed over the shots for trace cotangents (one nt x nrecs matrix per shot)."
 function fact_gradient(plan::HIPPlan, trace_bar::Vector{<:AbstractMatrix})
     flat = reduce(vcat, [vec(Matrix{Float64}(t)) for t in trace_bar])
