@@ -435,7 +435,9 @@ def profile_kernel(S, dtype, nshots, tiling, launches, time_block=None):
         P.set_time_block(2, *time_block[:3])
         steps = 2
     P.reset()
-    ms = (P.time_stencil2(launches) if steps == 2 else P.time_stencil(launches)) / launches
+    timer = P.time_stencil2 if steps == 2 else P.time_stencil
+    timer(1)  # loads the kernel's code object outside the timed launches
+    ms = timer(launches) / launches
     esize = 8 if dtype == "f64" else 4
     B, _ = algorithmic_bytes(S.n, S.halo, True, esize, nshots)
     B *= steps
