@@ -157,6 +157,7 @@ struct awcpml_plan {
     int64_t grad_fold = 0;
     int adj_group = 0;      // shots per thread of the reverse step, 0: automatic
     void* q_cur = nullptr;  // d(pnew)/d(fact) output of the next stencil launch
+    void* q_next = nullptr; // and of its second step (two-step launches)
 
     // two time steps per launch (3D): step t+2's field, psi and xi go to
     // spare buffers, which then trade places with the buffers they replace
@@ -482,10 +483,10 @@ int do_step(awcpml_plan* P, int64_t it0) {
 
 // Two-step launches can run: spare buffers allocated, every field buffer
 // zero on the Dirichlet nodes, at most kMaxSrc2 sources per shot (staged
-// in LDS), 32-bit byte offsets, no d(pnew)/d(fact) output.  two_ok: they
-// also were requested for the plan.
+// in LDS), 32-bit byte offsets, and a d(pnew)/d(fact) output (if any) for
+// both steps.  two_ok: they also were requested for the plan.
 bool two_applies(const awcpml_plan* P) {
-    return P->p_spare && !P->q_cur && !P->src_on_boundary &&
+    return P->p_spare && (!P->q_cur || P->q_next) && !P->src_on_boundary &&
            P->src_per_shot_max <= awcpml::kMaxSrc2 && !P->boundary_dirty && !P->wide &&
            (uint64_t)P->field_elems * P->esize < (1ull << 32);
 }
@@ -573,6 +574,8 @@ awcpml::Stencil2Args<T> make_args2(const awcpml_plan* P, int64_t it0, int W, int
     A.grid_blocks = (int)round_up(A.logical_blocks, awcpml::kNumXcd);
     B.p1 = (T*)P->p[(P->rot + 2) % 3];
     B.p2 = (T*)P->p_spare;
+    B.q1 = (T*)P->q_cur;
+    B.q2 = (T*)P->q_next;
     B.src_lin = (const int64_t*)P->d_src_lin;
     B.src_wav = (const T*)P->d_src_wav;
     B.src_start = (const int*)P->d_src_start;
@@ -2144,8 +2147,8 @@ struct AdjBuffers {
 // cp = false: the launch over block class 0 (no CPML nodes); true: over
 // classes 1..7.
 template <typename T>
-awcpml::AdjointArgs<T> make_adj_args(const awcpml_plan* P, const AdjBuffers& Bf, int lr, int par,
-                                     const void* q, bool cp) {
+awcpml::AdjointArgs<T> make_adj_args(const awcpml_plan* P, const AdjBuffers& Bf, const AdjShape& S,
+                                     int lr, int par, const void* q, bool cp) {
     awcpml::AdjointArgs<T> A;
     memset(&A, 0, sizeof A);
     for (int a = 0; a < 3; ++a) A.n[a] = P->n[a];
@@ -2179,7 +2182,6 @@ awcpml::AdjointArgs<T> make_adj_args(const awcpml_plan* P, const AdjBuffers& Bf,
     }
     A.ndim = P->ndim;
     A.nshots = P->nshots;
-    const AdjShape S = adj_shape(P);
     A.group = cp ? S.group_cp : S.group;
     A.tiles_x = S.tiles_x;
     A.tiles_y = S.tiles_y;
@@ -2341,18 +2343,22 @@ int adjoint_t(awcpml_plan* P, const double* tbar, double* fbar) {
     for (int64_t seg = P->ck_count - 1; seg >= 0; --seg) {
         const int64_t c0 = seg * K, c1 = std::min<int64_t>(c0 + K, nt);
         if ((st = ck_copy(P, seg, false))) return st;
-        for (int64_t s = c0 + 1; s <= c1; ++s) {  // replay, storing q^s
+        for (int64_t s = c0 + 1; s <= c1;) {  // replay, storing q^s
             P->q_cur = (char*)Bf.q + (size_t)(s - c0 - 1) * fb;
-            st = do_step(P, s - 1);
-            P->q_cur = nullptr;
+            P->q_next = s + 1 <= c1 ? (char*)Bf.q + (size_t)(s - c0) * fb : nullptr;
+            const int64_t n = P->q_next && two_ok(P) ? 2 : 1;  // bit-identical q either way
+            if (n == 1) P->q_next = nullptr;
+            st = n == 2 ? do_step2(P, s - 1) : do_step(P, s - 1);
+            P->q_cur = P->q_next = nullptr;
             if (st) return st;
-            ++P->it;
+            P->it += n;
+            s += n;
         }
         for (int64_t s = c1; s > c0; --s) {
             const void* q = (const char*)Bf.q + (size_t)(s - c0 - 1) * fb;
             awcpml::AdjointArgs<T> A;
             for (int cp = 0; cp < 2; ++cp) {  // blocks without, then with CPML nodes
-                A = make_adj_args<T>(P, Bf, lr, par, q, cp != 0);
+                A = make_adj_args<T>(P, Bf, shape, lr, par, q, cp != 0);
                 if (A.logical_blocks == 0) continue;
                 const dim3 grid(A.grid_blocks), block(awcpml::kTileX, kAdjBlockY);
                 if (P->ndim == 3) {
@@ -2437,10 +2443,9 @@ int awcpml_checkpoint_run(awcpml_plan* P, int64_t every) {
     for (int64_t c = 0; c < count; ++c) {
         if ((st = ck_copy(P, c, true))) return st;
         const int64_t n = std::min<int64_t>(every, P->nt - P->it);
-        for (int64_t q = 0; q < n; ++q) {
-            if ((st = do_step(P, P->it))) return st;
-            ++P->it;
-        }
+        // two steps per launch where the plan takes them (bit-identical)
+        if ((st = enqueue_steps(P, P->it, n))) return st;
+        P->it += n;
     }
     HIP_TRY(hipStreamSynchronize(P->stream));
     P->ck_valid = true;
@@ -2486,7 +2491,7 @@ int awcpml_adjoint(awcpml_plan* P, const double* trace_bar, int64_t count, doubl
                     (long long)count, (long long)(P->n_rec_total * P->nt));
     st = P->dtype == AWCPML_F64 ? adjoint_t<double>(P, trace_bar, fact_bar)
                                 : adjoint_t<float>(P, trace_bar, fact_bar);
-    P->q_cur = nullptr;
+    P->q_cur = P->q_next = nullptr;
     return st;
 }
 

@@ -26,8 +26,10 @@
 //
 // The host uses this kernel only when every field buffer is zero on the
 // Dirichlet nodes (no source on such a node since the last reset; the halo
-// lanes and rows read those nodes as 0), the fields take 32-bit byte
-// offsets and no d(pnew)/d(fact) output is requested.
+// lanes and rows read those nodes as 0) and the fields take 32-bit byte
+// offsets.  The optional d(pnew)/d(fact) outputs of both steps (the
+// adjoint's replay) are the single-step kernel's q, operation for
+// operation.
 #pragma once
 
 #include <utility>
@@ -47,6 +49,10 @@ struct Stencil2Args {
     StencilArgs<T> s;
     T* p1;                   // p^{t+1}
     T* p2;                   // p^{t+2}
+    // optional: d(pnew)/d(fact) of steps t+1 and t+2 (as StencilArgs::qout,
+    // for the adjoint's replay), null = off
+    T* q1;
+    T* q2;
     T* xi_out[3][2];         // xi after step t+2
     const int64_t* src_lin;  // element offsets of the sources (all shots)
     const T* src_wav;        // nt values per source
@@ -113,6 +119,7 @@ __device__ __forceinline__ double own_reg(double v) {
 template <typename T>
 struct Mid {
     T p1;           // p^{t+1}
+    T q;            // d(p^{t+1})/d(fact): laplacian + sum of (dpsi + xi)
     T pc, f;        // p^t (step t+2's pold) and fact
     T xl, xh, xx;   // x slab: psi^{t+1} of edges m-1 and m, xi^{t+1}[m-1]
     T yl, yh, yx;   // y slab
@@ -156,9 +163,11 @@ __device__ __forceinline__ Mid<T> two_first(const March<T, Off, 3>& M, int mz,
     const T d2b = (yp - two * pc + ym) * M.i2b;
     const T d2m = (q.pp - two * pc + pm) * M.i2m;
     T damp = T(0);
+    T slab = T(0);  // sum of (dpsi + xi), for the q outputs
     if (SA == 2 && M.sa.in) {
         const Pair<T> e = cpml_pair(M.sa.c, q.psa0, q.psa1, q.xia, xm, pc, xp, d2a, M.ia);
         damp += f * e.t;
+        slab += e.t;
         r.xl = e.lo;
         r.xh = e.hi;
         r.xx = e.xn;
@@ -166,6 +175,7 @@ __device__ __forceinline__ Mid<T> two_first(const March<T, Off, 3>& M, int mz,
     if (SB) {
         const Pair<T> e = cpml_pair(M.sb.c, q.psb0, q.psb1, q.xib, ym, pc, yp, d2b, M.ib);
         damp += f * e.t;
+        slab += e.t;
         r.yl = e.lo;
         r.yh = e.hi;
         r.yx = e.xn;
@@ -174,12 +184,14 @@ __device__ __forceinline__ Mid<T> two_first(const March<T, Off, 3>& M, int mz,
         const LayerCoef<T> cz = lds_coef(M.lds, M.h1, ZS, mz);
         const Pair<T> e = cpml_pair(cz, q.psm0, q.psm1, q.xim, pm, pc, q.pp, d2m, M.imm);
         damp += f * e.t;
+        slab += e.t;
         r.zl = e.lo;
         r.zh = e.hi;
         r.zx = e.xn;
     }
     const T lap = d2a + d2b + d2m;  // (d2x + d2y) + d2z
     r.p1 = two * pc - q.po + f * lap + damp;
+    r.q = lap + slab;
     r.pc = pc;
     r.f = own_reg(f);
     return r;
@@ -190,6 +202,8 @@ template <typename T, typename Off>
 struct TwoOut {
     T* p1;
     T* p2;
+    T* q1;        // optional d(pnew)/d(fact) outputs of the two steps
+    T* q2;
     T* xi_x;      // this column's x slab side (valid when M.sa.in)
     T* xi_y;      // this row's y slab side (valid when M.sb.in)
     T* xi_z[2];   // z slab sides
@@ -207,9 +221,11 @@ __device__ __forceinline__ void two_second(const March<T, Off, 3>& M, const TwoO
     const T d2b = (yp - two * pc + ym) * M.i2b;
     const T d2m = (pp - two * pc + pm) * M.i2m;
     T damp = T(0);
+    T slab = T(0);  // sum of (dpsi + xi), for the q outputs
     if (SA == 2 && M.sa.in) {
         const Pair<T> e = cpml_pair(M.sa.c, d.xh, d.xl, d.xx, xm, pc, xp, d2a, M.ia);
         damp += f * e.t;
+        slab += e.t;
         if (own) {
             stg(M.sa.p.psi_wr, M.sa.o_psi + (Off)k * M.sa.st_psi, e.hi);
             if (M.sa.first) stg(M.sa.p.psi_wr, M.sa.o_psi1 + (Off)k * M.sa.st_psi, e.lo);
@@ -219,6 +235,7 @@ __device__ __forceinline__ void two_second(const March<T, Off, 3>& M, const TwoO
     if (SB) {
         const Pair<T> e = cpml_pair(M.sb.c, d.yh, d.yl, d.yx, ym, pc, yp, d2b, M.ib);
         damp += f * e.t;
+        slab += e.t;
         if (own) {
             stg(M.sb.p.psi_wr, M.sb.o_psi + (Off)k * M.sb.st_psi, e.hi);
             if (M.sb.first) stg(M.sb.p.psi_wr, M.sb.o_psi1 + (Off)k * M.sb.st_psi, e.lo);
@@ -229,6 +246,7 @@ __device__ __forceinline__ void two_second(const March<T, Off, 3>& M, const TwoO
         const LayerCoef<T> cz = lds_coef(M.lds, M.h1, ZS, mz);
         const Pair<T> e = cpml_pair(cz, d.zh, d.zl, d.zx, pm, pc, pp, d2m, M.imm);
         damp += f * e.t;
+        slab += e.t;
         if (own) {
             T* pw = M.pm_side[ZS < 0 ? 0 : ZS].psi_wr;
             stg(pw, M.col + (Off)mz * M.sm, e.hi);
@@ -238,6 +256,7 @@ __device__ __forceinline__ void two_second(const March<T, Off, 3>& M, const TwoO
     }
     const T lap = d2a + d2b + d2m;
     if (own) stg_stream(O.p2, M.col + (Off)k * M.sm, two * pc - d.pc + f * lap + damp);
+    if (own && O.q2) stg(O.q2, M.col + (Off)k * M.sm, lap + slab);
 }
 
 // Step t+1 loads of march position k into a ring slot.  The wave's own
@@ -427,8 +446,10 @@ __device__ __forceinline__ void two_iter(const Stencil2Args<T>& B, const March<T
     // Dirichlet row), and p^{t+1} at plane k
     pb[C.so_n + kTileX] = G.iny ? pp : T(0);
     rb[C.so_c + kTileX] = dn.p1;
-    if (G.own && (!GEN || (k >= G.kbeg && k < G.kend)))
+    if (G.own && (!GEN || (k >= G.kbeg && k < G.kend))) {
         stg_stream(O.p1, M.col + (Off)k * M.sm, dn.p1);
+        if (O.q1) stg(O.q1, M.col + (Off)k * M.sm, dn.q);
+    }
     __syncthreads();
     // step t+2 on the owned rows only: the first and last wave are the
     // block's halo rows, which store nothing (a wave-uniform branch that
@@ -586,6 +607,8 @@ __device__ __forceinline__ void stencil3d_two_block(const Stencil2Args<T>& B, in
     TwoOut<T, Off> O;
     O.p1 = B.p1 + fo;
     O.p2 = B.p2 + fo;
+    O.q1 = B.q1 ? B.q1 + fo : nullptr;
+    O.q2 = B.q2 ? B.q2 + fo : nullptr;
     int mx, my;
     const int sx = classify(ic, nx, halo, A.active[0], &mx);
     const int sy = classify(jc, ny, halo, A.active[1], &my);
@@ -664,9 +687,11 @@ __device__ __forceinline__ Mid<T> two2_first(const March<T, Off, 2>& M, int mz,
     const T d2a = (xp - two * pc + xm) * M.i2a;
     const T d2m = (q.pp - two * pc + pm) * M.i2m;
     T damp = T(0);
+    T slab = T(0);  // sum of (dpsi + xi), for the q outputs
     if (SA == 2 && M.sa.in) {
         const Pair<T> e = cpml_pair(M.sa.c, q.psa0, q.psa1, q.xia, xm, pc, xp, d2a, M.ia);
         damp += f * e.t;
+        slab += e.t;
         r.xl = e.lo;
         r.xh = e.hi;
         r.xx = e.xn;
@@ -675,12 +700,14 @@ __device__ __forceinline__ Mid<T> two2_first(const March<T, Off, 2>& M, int mz,
         const LayerCoef<T> cz = lds_coef(M.lds, M.h1, ZS, mz);
         const Pair<T> e = cpml_pair(cz, q.psm0, q.psm1, q.xim, pm, pc, q.pp, d2m, M.imm);
         damp += f * e.t;
+        slab += e.t;
         r.zl = e.lo;
         r.zh = e.hi;
         r.zx = e.xn;
     }
     const T lap = d2a + d2m;  // d2x + d2y
     r.p1 = two * pc - q.po + f * lap + damp;
+    r.q = lap + slab;
     r.pc = pc;
     r.f = own_reg(f);
     return r;
@@ -696,9 +723,11 @@ __device__ __forceinline__ void two2_second(const March<T, Off, 2>& M, const Two
     const T d2a = (xp - two * pc + xm) * M.i2a;
     const T d2m = (pp - two * pc + pm) * M.i2m;
     T damp = T(0);
+    T slab = T(0);  // sum of (dpsi + xi), for the q outputs
     if (SA == 2 && M.sa.in) {
         const Pair<T> e = cpml_pair(M.sa.c, d.xh, d.xl, d.xx, xm, pc, xp, d2a, M.ia);
         damp += f * e.t;
+        slab += e.t;
         if (own) {
             stg(M.sa.p.psi_wr, M.sa.o_psi + (Off)k * M.sa.st_psi, e.hi);
             if (M.sa.first) stg(M.sa.p.psi_wr, M.sa.o_psi1 + (Off)k * M.sa.st_psi, e.lo);
@@ -709,6 +738,7 @@ __device__ __forceinline__ void two2_second(const March<T, Off, 2>& M, const Two
         const LayerCoef<T> cz = lds_coef(M.lds, M.h1, ZS, mz);
         const Pair<T> e = cpml_pair(cz, d.zh, d.zl, d.zx, pm, pc, pp, d2m, M.imm);
         damp += f * e.t;
+        slab += e.t;
         if (own) {
             T* pw = M.pm_side[ZS < 0 ? 0 : ZS].psi_wr;
             stg(pw, M.col + (Off)mz * M.sm, e.hi);
@@ -718,6 +748,7 @@ __device__ __forceinline__ void two2_second(const March<T, Off, 2>& M, const Two
     }
     const T lap = d2a + d2m;
     if (own) stg_stream(O.p2, M.col + (Off)k * M.sm, two * pc - d.pc + f * lap + damp);
+    if (own && O.q2) stg(O.q2, M.col + (Off)k * M.sm, lap + slab);
 }
 
 // Step t+1 loads of row k (2D): pp = pcur at row k+1 (every row below the
@@ -814,8 +845,10 @@ __device__ __forceinline__ void two2_iter(const Stencil2Args<T>& B, const March<
     const int kr = k + PF;
     if (!GEN || kr <= G.kend) C.qr[U] = two2_load<T, Off, SA, ZM, GEN>(G, M, kr);
     __builtin_amdgcn_sched_barrier(0);
-    if (G.own && (!GEN || (k >= G.kbeg && k < G.kend)))
+    if (G.own && (!GEN || (k >= G.kbeg && k < G.kend))) {
         stg_stream(O.p1, M.col + (Off)k * M.sm, dn.p1);
+        if (O.q1) stg(O.q1, M.col + (Off)k * M.sm, dn.q);
+    }
     if (!GEN || k - 1 >= G.kbeg) {
         const T xm = lane_shift(C.d.p1, true);
         const T xp = lane_shift(C.d.p1, false);
@@ -928,6 +961,8 @@ __device__ __forceinline__ void stencil2d_two_block(const Stencil2Args<T>& B, in
     TwoOut<T, Off> O;
     O.p1 = B.p1 + fo;
     O.p2 = B.p2 + fo;
+    O.q1 = B.q1 ? B.q1 + fo : nullptr;
+    O.q2 = B.q2 ? B.q2 + fo : nullptr;
     int mx;
     const int sx = classify(ic, nx, halo, A.active[0], &mx);
     O.xi_x = (sx == 1 ? B.xi_out[0][1] : B.xi_out[0][0]) + shot * A.xi_stride[0];
