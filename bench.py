@@ -538,7 +538,8 @@ def main():
                     help="profiler target: stencil launches of the 64-shot 1066^2 2D workload (FP32); "
                          "--tiling block_y,march_len,prefetch,shot_group, --time-block for two-step")
     ap.add_argument("--profile-fwi", action="store_true",
-                    help="profiler target: one FP32 FWI gradient of the 554^2 32-shot workload")
+                    help="profiler target: one FP32 FWI gradient of the 554^2 32-shot workload; "
+                         "--time-block rows,march_len,prefetch for two-step forward launches")
     ap.add_argument("--profile-uneven", action="store_true",
                     help="profiler target: batch stencil launches of the 12-model uneven 3D "
                          "workload (FP32); --time-block rows,march_len,prefetch for two-step "
@@ -560,6 +561,8 @@ def main():
         fc = fwi2d()
         FS = _setup(fc)
         P = _shot_plan(FS, fc["shots"], "f32")
+        if args.time_block:  # two-step launches for both forward passes
+            P.set_time_block(2, *(int(v) for v in args.time_block.split(",")))
         P.reset()
         P.advance(FS.nt)
         obs = [P.traces(q) for q in range(P.nshots)]
