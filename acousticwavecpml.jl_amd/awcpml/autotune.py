@@ -42,7 +42,7 @@ def two_step_candidates(plan):
     """(block_rows, march_len, prefetch) of two-step launches worth timing."""
     if plan.ndim == 2:
         n_march = plan.n[1] - 2
-        return [(w, min(n_march, ml), pf) for w in (4, 8) for ml in (32, 64, 128)
+        return [(w, min(n_march, ml), pf) for w in (4, 8) for ml in (16, 32, 64, 128)
                 for pf in (1, 2, 3)]
     n_march = plan.n[2] - 2
     out = []

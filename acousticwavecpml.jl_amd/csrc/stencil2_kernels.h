@@ -891,6 +891,9 @@ __device__ __forceinline__ void two2_march(const Stencil2Args<T>& B, const March
     two2_group<T, Off, PF, SA, ZM, true, true>(B, M, O, G, S, C, k, std::make_integer_sequence<int, H>{});
     k += H;
     const int last = (G.kend < G.nz - 2 ? G.kend : G.nz - 2) - PF;
+    // drain the head's memory operations once, as in two_march, so that the
+    // steady loop's waits follow its own issue order
+    __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
     for (; k + PF - 1 <= last; k += PF)
         two2_group<T, Off, PF, SA, ZM, false, false>(B, M, O, G, S, C, k,
                                                      std::make_integer_sequence<int, PF>{});
