@@ -53,6 +53,8 @@ SIGNATURES = {
     "awcpml_set_time_block": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                              ctypes.c_int]),
     "awcpml_get_time_block": (ctypes.c_int, [_vp] + [ctypes.POINTER(ctypes.c_int)] * 4),
+    "awcpml_set_two_step_align": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "awcpml_get_two_step_align": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "awcpml_time_stencil2": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "awcpml_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
     "awcpml_set_wide_offsets": (ctypes.c_int, [_vp, ctypes.c_int]),

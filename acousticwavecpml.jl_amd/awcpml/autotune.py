@@ -72,6 +72,8 @@ def _apply(plan, best):
     plan.set_shot_group(best.get("shot_group", 1))
     tb = best.get("time_block")
     if tb:
+        if "align" in tb:
+            plan.set_two_step_align(tb["align"])
         plan.set_time_block(2, tb["block_rows"], tb["march_len"], tb["prefetch"])
     else:
         plan.set_time_block(1)
@@ -118,18 +120,28 @@ def autotune(plan, cands=None, launches=8, cache=DEFAULT_CACHE, use_cache=True):
     # two time steps per launch, compared per time step
     from . import _lib
     two = []
+    # 3D: each shape with aligned 64-node tiles and with 62-node tiles
+    aligns = (True, False) if plan.ndim == 3 else (None,)
+    align0 = plan.two_step_align() if plan.ndim == 3 else None
     for w, ml2, pf2 in (two_step_candidates(plan) if cands is None else []):
         try:
-            plan.set_time_block(2, w, ml2, pf2)
-            plan.reset()
-            plan.time_stencil2(1)  # warm-up
-            n2 = max(1, launches // 2)
-            two.append((plan.time_stencil2(n2) / n2 / 2.0, w, ml2, pf2))
+            for al in aligns:
+                if al is not None:
+                    plan.set_two_step_align(al)
+                plan.set_time_block(2, w, ml2, pf2)
+                plan.reset()
+                plan.time_stencil2(1)  # warm-up
+                n2 = max(1, launches // 2)
+                two.append((plan.time_stencil2(n2) / n2 / 2.0, w, ml2, pf2, al))
         except _lib.AwcpmlError:
             break  # two-step launches do not apply to this plan
+    if align0 is not None:
+        plan.set_two_step_align(align0)
     if two and min(two)[0] < best["ms_per_step"]:
-        ms2, w, ml2, pf2 = min(two)
+        ms2, w, ml2, pf2, al = min(two, key=lambda t: t[0])
         best["time_block"] = {"block_rows": w, "march_len": ml2, "prefetch": pf2}
+        if al is not None:
+            best["time_block"]["align"] = al
         best["ms_per_step"] = ms2
     best["tried"] += len(two)
     _apply(plan, best)

@@ -137,6 +137,16 @@ class Plan:
         _lib.check(_lib.load().awcpml_get_time_block(self._h, *[ctypes.byref(x) for x in v]))
         return tuple(x.value for x in v)
 
+    def set_two_step_align(self, enable=True):
+        """3D two-step launches: aligned 64-node x tiles for the blocks no
+        CPML slab reaches (default on); results are bit-identical either way."""
+        _lib.check(_lib.load().awcpml_set_two_step_align(self._h, int(bool(enable))))
+
+    def two_step_align(self):
+        v = ctypes.c_int()
+        _lib.check(_lib.load().awcpml_get_two_step_align(self._h, ctypes.byref(v)))
+        return bool(v.value)
+
     # -- execution ----------------------------------------------------
     def reset(self):
         _lib.check(_lib.load().awcpml_reset(self._h))

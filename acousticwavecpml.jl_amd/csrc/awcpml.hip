@@ -164,6 +164,7 @@ struct awcpml_plan {
     int time_block = 1;
     int tb_rows = 0, tb_march = 0;   // waves per block, chunk length (0: default)
     int tb_prefetch = 0;             // planes of loads in flight (0: default)
+    int tb_align = 1;                // 3D: aligned 64-lane tiles where no CPML slab reaches (FP32 default)
     void* p_spare = nullptr;
     void* psi_spare[3][2] = {};
     void* xi_spare[3][2] = {};
@@ -541,6 +542,20 @@ int two_chunk_list(const awcpml_plan* P, int ml, int* starts) {
     return count;
 }
 
+// Aligned 64-lane tiles need the plain stencil at the nodes just outside
+// each tile (x = 64*tx-1 and 64*tx+64): none of them may be an interior
+// node of an active x slab.
+bool two_align_ok(const awcpml_plan* P) {
+    const int64_t nx = P->n[0], h = P->halo;
+    const int64_t tiles = (nx + awcpml::kTileX - 1) / awcpml::kTileX;
+    for (int64_t t = 0; t < tiles; ++t)
+        for (int64_t x : {t * awcpml::kTileX - 1, t * awcpml::kTileX + awcpml::kTileX}) {
+            if (x < 1 || x > nx - 2) continue;
+            if ((P->active[0][0] && x <= h) || (P->active[0][1] && x >= nx - h - 1)) return false;
+        }
+    return true;
+}
+
 // Arguments of a two-step launch from the current layout: step t+1 reads
 // p[rot] (pold), p[rot+1] (pcur), psi[par] and xi and writes p[rot+2];
 // step t+2 writes the spare field, psi and xi buffers.
@@ -561,6 +576,10 @@ awcpml::Stencil2Args<T> make_args2(const awcpml_plan* P, int64_t it0, int W, int
     A.tiles_x = (int)((P->n[0] - 2 + awcpml::kOwnX - 1) / awcpml::kOwnX);
     A.march_len = ml;
     B.nchunk_list = two_chunk_list(P, ml, B.chunk_start);
+    B.tiles62 = A.tiles_x;
+    B.tiles64 = (int)((P->n[0] + awcpml::kTileX - 1) / awcpml::kTileX);
+    B.align = P->ndim == 3 && P->tb_align && two_align_ok(P);
+    if (B.align) A.tiles_x = std::max(B.tiles62, B.tiles64);
     if (P->ndim == 3) {
         A.tiles_y = (int)((P->n[1] - 2 + W - 3) / (W - 2));
         A.nchunks = B.nchunk_list > 0 ? B.nchunk_list : (int)((P->n[2] - 2 + ml - 1) / ml);
@@ -1229,6 +1248,9 @@ int awcpml_plan_create(int ndim, int64_t nx, int64_t ny, int64_t nz, int halo, i
     P->halo = halo;
     for (int a = 0; a < 3; ++a) P->n[a] = n[a];
     P->esize = dtype == AWCPML_F64 ? 8 : 4;
+    // aligned two-step tiles: on for FP32 (whole-line stores pay for the
+    // edge nodes' extra loads); FP64 measured faster with 62-node tiles
+    P->tb_align = dtype == AWCPML_F32;
     P->px = round_up(nx, (int64_t)(128 / P->esize));
     P->field_elems = P->px * ny * nz;
     P->src_lin.resize(nshots);
@@ -1674,6 +1696,22 @@ int awcpml_get_time_block(const awcpml_plan* P, int* steps, int* block_rows, int
     *block_rows = W;
     *march_len = ml;
     *prefetch = pf;
+    return AWCPML_OK;
+}
+
+int awcpml_set_two_step_align(awcpml_plan* P, int enable) {
+    g_err.clear();
+    int st = check_plan(P);
+    if (st) return st;
+    P->tb_align = enable != 0;
+    drop_graphs(P->graphs);
+    ++P->gen;
+    return AWCPML_OK;
+}
+
+int awcpml_get_two_step_align(const awcpml_plan* P, int* enable) {
+    if (!P || !enable) return fail(AWCPML_ERR_ARG, "NULL argument");
+    *enable = P->tb_align;
     return AWCPML_OK;
 }
 

@@ -24,6 +24,20 @@
 // step is the single-step kernel's, operation for operation, so two steps
 // of this kernel equal two single-step launches bit for bit.
 //
+// Aligned tiles.  Field rows start on 128-byte boundaries, but the 62 owned
+// lanes of a wave do not: every row a wave stores then begins and ends in
+// cache lines that a neighbouring tile's block completes at some other
+// time, and such partial-line writes cost HBM more than twice what whole
+// lines cost.  Where no CPML slab reaches a block's rows, its chunk's
+// planes or the x nodes next to its tile, the block instead owns all 64
+// lanes of a tile starting at x = 64*tx (lanes outside the interior store
+// 0, the Dirichlet value, so every stored row is whole lines).  Step t+2 on
+// lanes 0 and 63 then needs p^{t+1} at x = 64*tx-1 and 64*tx+64: the two
+// halo-row waves, which have no step t+2 of their own, evaluate step t+1 at
+// those nodes for every row of the block (one row per lane) and pass the
+// values through a small LDS ring.  Ownership is decided per row group and
+// chunk, so the two tilings partition the grid between them.
+//
 // The host uses this kernel only when every field buffer is zero on the
 // Dirichlet nodes (no source on such a node since the last reset; the halo
 // lanes and rows read those nodes as 0) and the fields take 32-bit byte
@@ -59,6 +73,12 @@ struct Stencil2Args {
     const int* src_start;    // nshots+1 CSR offsets
     int64_t nt;
     int64_t it0;             // 0-based index of step t+1
+    // 3D: 1 = blocks without CPML in their rows and planes take aligned
+    // 64-lane tiles (the host has checked that no tile's outside x
+    // neighbours lie in an x slab); tiles_x then counts the larger of the
+    // two tilings' tiles
+    int align;
+    int tiles62, tiles64;    // x tiles of the 62-lane and the aligned tiling
     // march chunks: chunk c owns planes (2D: rows) [chunk_start[c],
     // chunk_start[c+1]); 0 chunks listed: chunk c starts at 1 + c*march_len.
     // The host cuts the CPML layers of the march axis into chunks of their
@@ -207,11 +227,17 @@ struct TwoOut {
     T* xi_x;      // this column's x slab side (valid when M.sa.in)
     T* xi_y;      // this row's y slab side (valid when M.sb.in)
     T* xi_z[2];   // z slab sides
+    // aligned tiles: this lane stores p^{t+1} and p^{t+2} (an owned row, x
+    // inside the row pitch) at byte offset scol of plane 0 (its own x, not
+    // the clamped one of its loads), and its node is interior (else the
+    // stored value is 0)
+    bool st, in1;
+    Off scol;
 };
 
 // Step t+2 at one node (plane k) from its step t+1 results d and the
 // neighbouring p^{t+1} values; stores on owned nodes only.
-template <typename T, typename Off, int SA, bool SB, int ZS>
+template <typename T, typename Off, int SA, bool SB, int ZS, bool AL>
 __device__ __forceinline__ void two_second(const March<T, Off, 3>& M, const TwoOut<T, Off>& O,
                                            const Mid<T>& d, T pm, T pp, T xm, T xp, T ym, T yp,
                                            bool own, int mz, int k) {
@@ -255,8 +281,15 @@ __device__ __forceinline__ void two_second(const March<T, Off, 3>& M, const TwoO
         }
     }
     const T lap = d2a + d2b + d2m;
-    if (own) stg_stream(O.p2, M.col + (Off)k * M.sm, two * pc - d.pc + f * lap + damp);
-    if (own && O.q2) stg(O.q2, M.col + (Off)k * M.sm, lap + slab);
+    if (AL) {
+        // whole rows: lanes outside the interior store the Dirichlet 0
+        const T v = two * pc - d.pc + f * lap + damp;
+        if (O.st) stg_stream(O.p2, O.scol + (Off)k * M.sm, O.in1 ? v : T(0));
+        if (O.st && O.q2) stg(O.q2, O.scol + (Off)k * M.sm, O.in1 ? lap + slab : T(0));
+    } else {
+        if (own) stg_stream(O.p2, M.col + (Off)k * M.sm, two * pc - d.pc + f * lap + damp);
+        if (own && O.q2) stg(O.q2, M.col + (Off)k * M.sm, lap + slab);
+    }
 }
 
 // Step t+1 loads of march position k into a ring slot.  The wave's own
@@ -318,6 +351,48 @@ struct TwoGeom {
     bool iny;     // the wave's y is its clamped y
     uint32_t dxm; // 3D: byte offset from the lane's node to the x neighbour it loads
     uint32_t dym; // 3D: byte offset from the wave's row to the y neighbour it loads
+    // aligned tiles: the nodes just outside the tile, x = 64*tx-1 (side 0)
+    // and 64*tx+64 (side 1).  Lanes 0 and 63 of every row load what step
+    // t+1 at their side's node needs beyond their own loads (its outer x
+    // neighbour, pold, fact) and pass it through LDS; lane r < W of the
+    // halo-row waves (wave 0: side 0, wave W-1: side 1) evaluates that node
+    // in the block's row r.
+    bool eln;         // lane 0 or 63 of an aligned tile
+    uint32_t dxe;     // byte offset from the lane's column to its side's node (clamped)
+    uint32_t dxo;     // ... and to that node's outer x neighbour
+    int64_t hcol_el;  // halo-row waves: element offset of lane r's node column (sources)
+    bool hin;         // ... that node is interior
+};
+
+// Step t+1 inputs of the node just outside an aligned tile that its row's
+// edge lane loads (see TwoGeom): the outer x neighbour of pcur, pold and
+// fact.  The node's pcur is the lane's x-neighbour load.
+template <typename T>
+struct EdgeSlot {
+    T xo, pe, fe;
+};
+
+template <typename T, typename Off, bool GEN>
+__device__ __forceinline__ EdgeSlot<T> edge_slot_load(const TwoGeom& G, const March<T, Off, 3>& M,
+                                                      int k) {
+    EdgeSlot<T> q = {};
+    if (GEN && (k < 1 || k > G.nz - 2)) return q;
+    if (G.eln) {
+        const Off c = M.col + (Off)k * M.sm;
+        q.xo = ldg(M.pcur, c + (Off)G.dxo);
+        q.pe = ldg(M.pold, c + (Off)G.dxe);
+        q.fe = ldg(M.fact, c + (Off)G.dxe);
+    }
+    return q;
+}
+
+// LDS of the aligned tiles' outside nodes: their pcur per row over four
+// planes (slot: plane & 3), and the other step t+1 inputs of plane k
+// (slot: k & 1) -- outer x neighbour, pold, fact, inner x neighbour.
+template <typename T, int W>
+struct EdgeLds {
+    T pc[4][2][W];
+    T in[2][4][2][W];
 };
 
 // ZM = false: no z slab layer in the chunk.  One code path for both sides
@@ -343,19 +418,19 @@ __device__ __forceinline__ Mid<T> two_first_at(const TwoGeom& G, const March<T, 
     return two_first<T, Off, SA, SB, -1>(M, mz, q, pm, pc, xm, xp, ym, yp);
 }
 
-template <typename T, typename Off, int SA, bool SB, bool ZM>
+template <typename T, typename Off, int SA, bool SB, bool ZM, bool AL>
 __device__ __forceinline__ void two_second_at(const TwoGeom& G, const March<T, Off, 3>& M,
                                               const TwoOut<T, Off>& O, const Mid<T>& d, T pm,
                                               T pp, T xm, T xp, T ym, T yp, bool own, int k) {
     if (!ZM) {
-        two_second<T, Off, SA, SB, -1>(M, O, d, pm, pp, xm, xp, ym, yp, own, 0, k);
+        two_second<T, Off, SA, SB, -1, AL>(M, O, d, pm, pp, xm, xp, ym, yp, own, 0, k);
         return;
     }
     int mz;
     const int zs = classify(k, G.nz, G.halo, G.act, &mz);
-    if (zs == 0) two_second<T, Off, SA, SB, 0>(M, O, d, pm, pp, xm, xp, ym, yp, own, mz, k);
-    else if (zs == 1) two_second<T, Off, SA, SB, 1>(M, O, d, pm, pp, xm, xp, ym, yp, own, mz, k);
-    else two_second<T, Off, SA, SB, -1>(M, O, d, pm, pp, xm, xp, ym, yp, own, mz, k);
+    if (zs == 0) two_second<T, Off, SA, SB, 0, AL>(M, O, d, pm, pp, xm, xp, ym, yp, own, mz, k);
+    else if (zs == 1) two_second<T, Off, SA, SB, 1, AL>(M, O, d, pm, pp, xm, xp, ym, yp, own, mz, k);
+    else two_second<T, Off, SA, SB, -1, AL>(M, O, d, pm, pp, xm, xp, ym, yp, own, mz, k);
 }
 
 // Per-wave march state: the ring of step t+1 loads (PF planes in flight),
@@ -370,6 +445,10 @@ struct TwoCtx {
     int slot;
     // 3D: element offsets of the LDS ring slots of planes k-1, k, k+1
     int so_p, so_c, so_n;
+    // aligned tiles, edge lanes: their side node's loads, PF planes in flight
+    EdgeSlot<T> es[PF];
+    // element offsets of the edge ring's slots of planes k-1, k, k+1
+    int eo_p, eo_c, eo_n;
 };
 
 // The shot's sources for step t+1, staged in LDS at the start of a launch
@@ -401,11 +480,13 @@ __device__ __forceinline__ int two_stage_sources(const Stencil2Args<T>& B, int64
 // plane k+PF, the LDS exchange, step t+2 at plane k-1.  GEN = false: k is
 // in the steady state, where every plane index condition holds (computed,
 // refilled, owned plane, step t+2 due), so none is tested.
-template <typename T, typename Off, int W, int PF, int SA, bool SB, bool ZM, bool GEN, int U>
+template <typename T, typename Off, int W, int PF, int SA, bool SB, bool ZM, bool AL, bool GEN,
+          int U>
 __device__ __forceinline__ void two_iter(const Stencil2Args<T>& B, const March<T, Off, 3>& M,
                                          const TwoOut<T, Off>& O, T (*ring)[W + 2][kTileX],
-                                         T (*pring)[W + 2][kTileX], const TwoGeom& G,
-                                         const TwoSrc<T>& S, TwoCtx<T, PF>& C, int k) {
+                                         T (*pring)[W + 2][kTileX], T (*eg)[2][W],
+                                         EdgeLds<T, W>* EL, const TwoGeom& G, const TwoSrc<T>& S,
+                                         TwoCtx<T, PF>& C, int k) {
     const int lane = (int)threadIdx.x;
     // this lane's column in the padded LDS rings, one row above its own:
     // slot c's rows wy-1, wy, wy+1 are at [so], [so + 64], [so + 128]
@@ -436,17 +517,42 @@ __device__ __forceinline__ void two_iter(const Stencil2Args<T>& B, const March<T
                 if (S.lin[e] == lin) dn.p1 += S.w[e];
         }
         dn.p1 = G.in1 ? dn.p1 : T(0);
+        if (AL && G.eln) {
+            // the side node's step t+1 inputs of plane k for the halo-row
+            // waves: outer x neighbour, pold, fact, and this lane's pcur
+            // (the node's inner x neighbour)
+            const int sd = lane == 0 ? 0 : 1;
+            EL->in[k & 1][0][sd][G.wy] = C.es[U].xo;
+            EL->in[k & 1][1][sd][G.wy] = C.es[U].pe;
+            EL->in[k & 1][2][sd][G.wy] = C.es[U].fe;
+            EL->in[k & 1][3][sd][G.wy] = pc;
+        }
+    }
+    if (AL && G.eln) {
+        // the side node's pcur at plane k+1 (this lane's x-neighbour load of
+        // the next ring slot; 0 on a Dirichlet row)
+        const T pn = C.qr[(U + 1) % PF].xm;
+        EL->pc[(k + 1) & 3][lane == 0 ? 0 : 1][G.wy] = G.iny ? pn : T(0);
     }
     C.pz0 = C.pz1;
     C.pz1 = pp;
     const int kr = k + PF;
-    if (!GEN || kr <= G.kend) C.qr[U] = two_load<T, Off, W, SA, SB, ZM, GEN>(G, M, kr, G.wy);
+    if (!GEN || kr <= G.kend) {
+        C.qr[U] = two_load<T, Off, W, SA, SB, ZM, GEN>(G, M, kr, G.wy);
+        if (AL) C.es[U] = edge_slot_load<T, Off, GEN>(G, M, kr);
+    }
     __builtin_amdgcn_sched_barrier(0);
     // this column's pcur at plane k+1 for the neighbouring rows (0 for a
     // Dirichlet row), and p^{t+1} at plane k
     pb[C.so_n + kTileX] = G.iny ? pp : T(0);
     rb[C.so_c + kTileX] = dn.p1;
-    if (G.own && (!GEN || (k >= G.kbeg && k < G.kend))) {
+    if (AL) {
+        // whole rows (dn.p1 is 0 outside the interior)
+        if (O.st && (!GEN || (k >= G.kbeg && k < G.kend))) {
+            stg_stream(O.p1, O.scol + (Off)k * M.sm, dn.p1);
+            if (O.q1) stg(O.q1, O.scol + (Off)k * M.sm, G.in1 ? dn.q : T(0));
+        }
+    } else if (G.own && (!GEN || (k >= G.kbeg && k < G.kend))) {
         stg_stream(O.p1, M.col + (Off)k * M.sm, dn.p1);
         if (O.q1) stg(O.q1, M.col + (Off)k * M.sm, dn.q);
     }
@@ -455,11 +561,48 @@ __device__ __forceinline__ void two_iter(const Stencil2Args<T>& B, const March<T
     // block's halo rows, which store nothing (a wave-uniform branch that
     // leaves their issue slots to the other waves)
     if ((!GEN || k - 1 >= G.kbeg) && G.wy >= 1 && G.wy <= W - 2) {
-        const T xm = lane_shift(C.d.p1, true);
-        const T xp = lane_shift(C.d.p1, false);
+        T xm = lane_shift(C.d.p1, true);
+        T xp = lane_shift(C.d.p1, false);
+        if (AL) {
+            // aligned tiles: lanes 0 and 63 take the nodes outside the tile
+            // from the edge ring (plane k-1)
+            const T* ep = &eg[0][0][0] + C.eo_p;
+            const T el = ep[G.wy], er = ep[W + G.wy];
+            xm = lane == 0 ? el : xm;
+            xp = lane == kTileX - 1 ? er : xp;
+        }
         const T ym = rb[C.so_p];                 // plane k-1, row wy-1
         const T yp = rb[C.so_p + 2 * kTileX];    // plane k-1, row wy+1
-        two_second_at<T, Off, SA, SB, ZM>(G, M, O, C.d, C.p1m, dn.p1, xm, xp, ym, yp, G.own, k - 1);
+        two_second_at<T, Off, SA, SB, ZM, AL>(G, M, O, C.d, C.p1m, dn.p1, xm, xp, ym, yp, G.own,
+                                              k - 1);
+    }
+    if (AL && (G.wy == 0 || G.wy == W - 1) && (!GEN || (k >= 1 && k <= G.nz - 2))) {
+        // aligned tiles: the halo-row waves evaluate step t+1 at plane k on
+        // their side's node outside the tile in every row of the block
+        // (lane r: row r), the plain stencil in the owning block's order of
+        // operations (no CPML slab reaches these nodes), from the inputs the
+        // rows' edge lanes put in LDS
+        const int sd = G.wy == 0 ? 0 : 1;
+        const int r = lane < W ? lane : W - 1;
+        const int rm = r > 0 ? r - 1 : 0, rp = r < W - 1 ? r + 1 : W - 1;
+        const T* pk = EL->pc[k & 3][sd];
+        const T hpc = pk[r], hym = pk[rm], hyp = pk[rp];
+        const T hpm = EL->pc[(k - 1) & 3][sd][r];
+        const T hpp = EL->pc[(k + 1) & 3][sd][r];
+        const T xo = EL->in[k & 1][0][sd][r];
+        const T inner = EL->in[k & 1][3][sd][r];
+        NodeLoads<T> q = {};
+        q.pp = hpp;
+        q.po = EL->in[k & 1][1][sd][r];
+        q.f = EL->in[k & 1][2][sd][r];
+        Mid<T> e = two_first<T, Off, 0, false, -1>(M, 0, q, hpm, hpc, sd ? inner : xo,
+                                                   sd ? xo : inner, hym, hyp);
+        if (k >= G.ksrc_lo && k <= G.ksrc_hi) {
+            const int64_t lin = G.hcol_el + (int64_t)k * G.plane;
+            for (int s = 0; s < G.ns; ++s)
+                if (S.lin[s] == lin) e.p1 += S.w[s];
+        }
+        if (lane < W) (&eg[0][0][0])[C.eo_c + sd * W + lane] = G.hin ? e.p1 : T(0);
     }
     C.p1m = C.d.p1;
     C.d = dn;
@@ -467,20 +610,27 @@ __device__ __forceinline__ void two_iter(const Stencil2Args<T>& B, const March<T
     C.so_p = C.so_c;
     C.so_c = C.so_n;
     C.so_n = so;
+    if (AL) {
+        const int eo = C.eo_p;
+        C.eo_p = C.eo_c;
+        C.eo_c = C.eo_n;
+        C.eo_n = eo;
+    }
 }
 
 // Positions k+u, u in Us..., in order, ring slot u % PF (a constant), each
 // only while k+u <= kend when CHECK.
-template <typename T, typename Off, int W, int PF, int SA, bool SB, bool ZM, bool GEN, bool CHECK,
-          int... Us>
+template <typename T, typename Off, int W, int PF, int SA, bool SB, bool ZM, bool AL, bool GEN,
+          bool CHECK, int... Us>
 __device__ __forceinline__ void two_group(const Stencil2Args<T>& B, const March<T, Off, 3>& M,
                                           const TwoOut<T, Off>& O, T (*ring)[W + 2][kTileX],
-                                          T (*pring)[W + 2][kTileX], const TwoGeom& G,
-                                          const TwoSrc<T>& S, TwoCtx<T, PF>& C, int k,
+                                          T (*pring)[W + 2][kTileX], T (*eg)[2][W],
+                                          EdgeLds<T, W>* EL, const TwoGeom& G, const TwoSrc<T>& S,
+                                          TwoCtx<T, PF>& C, int k,
                                           std::integer_sequence<int, Us...>) {
     ((!CHECK || k + Us <= G.kend
-          ? (two_iter<T, Off, W, PF, SA, SB, ZM, GEN, Us % PF>(B, M, O, ring, pring, G, S, C,
-                                                               k + Us),
+          ? (two_iter<T, Off, W, PF, SA, SB, ZM, AL, GEN, Us % PF>(B, M, O, ring, pring, eg, EL, G,
+                                                                   S, C, k + Us),
              0)
           : 0),
      ...);
@@ -492,11 +642,13 @@ __device__ __forceinline__ void two_group(const Stencil2Args<T>& B, const March<
 // the steady state unrolled PF times (constant slot indices), and a
 // generic tail.  Every wave of the block runs the same positions (kbeg,
 // kend are block-uniform) and so meets every barrier.
-template <typename T, typename Off, int W, int PF, int SA, bool SB, bool ZM>
+template <typename T, typename Off, int W, int PF, int SA, bool SB, bool ZM, bool AL>
 __device__ __forceinline__ void two_march(const Stencil2Args<T>& B, const March<T, Off, 3>& M,
                                           const TwoOut<T, Off>& O, T (*ring)[W + 2][kTileX],
-                                          T (*pring)[W + 2][kTileX], const TwoGeom& G,
+                                          T (*pring)[W + 2][kTileX], T (*eg)[2][W],
+                                          EdgeLds<T, W>* EL, const TwoGeom& G,
                                           const TwoSrc<T>& S) {
+    static_assert(!AL || PF >= 2, "aligned tiles read the next ring slot");
     TwoCtx<T, PF> C;
     C.pz0 = G.kbeg >= 2 ? ldg(M.pcur, M.col + (Off)(G.kbeg - 2) * M.sm) : T(0);
     C.pz1 = ldg(M.pcur, M.col + (Off)(G.kbeg - 1) * M.sm);
@@ -506,6 +658,15 @@ __device__ __forceinline__ void two_march(const Stencil2Args<T>& B, const March<
         C.qr[u] = kk <= G.kend ? two_load<T, Off, W, SA, SB, ZM, true>(G, M, kk, G.wy) : NodeLoads<T>{};
         __builtin_amdgcn_sched_barrier(0);
     }
+    if (AL) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int kk = G.kbeg - 1 + u;
+            C.es[u] = kk <= G.kend ? edge_slot_load<T, Off, true>(G, M, kk) : EdgeSlot<T>{};
+        }
+        // the side nodes' pcur at the first march position's plane
+        if (G.eln) EL->pc[(G.kbeg - 1) & 3][threadIdx.x == 0 ? 0 : 1][G.wy] = G.iny ? C.qr[0].xm : T(0);
+    }
     C.d = Mid<T>{};
     C.p1m = T(0);
     {
@@ -514,13 +675,17 @@ __device__ __forceinline__ void two_march(const Stencil2Args<T>& B, const March<
         C.so_c = s0 * SS;
         C.so_n = (s0 == 2 ? 0 : s0 + 1) * SS;
         C.so_p = (s0 == 0 ? 2 : s0 - 1) * SS;
+        constexpr int ES = 2 * W;  // elements per edge ring slot
+        C.eo_c = s0 * ES;
+        C.eo_n = (s0 == 2 ? 0 : s0 + 1) * ES;
+        C.eo_p = (s0 == 0 ? 2 : s0 - 1) * ES;
     }
     (&pring[0][0][0])[C.so_c + (G.wy + 1) * kTileX + threadIdx.x] = G.iny ? C.pz1 : T(0);
     __syncthreads();
     constexpr int H = (2 + PF - 1) / PF * PF;
     int k = G.kbeg - 1;
-    two_group<T, Off, W, PF, SA, SB, ZM, true, true>(B, M, O, ring, pring, G, S, C, k,
-                                                     std::make_integer_sequence<int, H>{});
+    two_group<T, Off, W, PF, SA, SB, ZM, AL, true, true>(B, M, O, ring, pring, eg, EL, G, S, C, k,
+                                                         std::make_integer_sequence<int, H>{});
     k += H;
     const int last = (G.kend < G.nz - 2 ? G.kend : G.nz - 2) - PF;  // last steady position
     // drain the head's memory operations once: the steady loop is then
@@ -529,11 +694,11 @@ __device__ __forceinline__ void two_march(const Stencil2Args<T>& B, const March<
     // head's different order they fall back to waiting for everything)
     __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
     for (; k + PF - 1 <= last; k += PF)
-        two_group<T, Off, W, PF, SA, SB, ZM, false, false>(B, M, O, ring, pring, G, S, C, k,
-                                                           std::make_integer_sequence<int, PF>{});
+        two_group<T, Off, W, PF, SA, SB, ZM, AL, false, false>(B, M, O, ring, pring, eg, EL, G, S, C,
+                                                               k, std::make_integer_sequence<int, PF>{});
     for (; k <= G.kend; k += PF)
-        two_group<T, Off, W, PF, SA, SB, ZM, true, true>(B, M, O, ring, pring, G, S, C, k,
-                                                         std::make_integer_sequence<int, PF>{});
+        two_group<T, Off, W, PF, SA, SB, ZM, AL, true, true>(B, M, O, ring, pring, eg, EL, G, S, C,
+                                                             k, std::make_integer_sequence<int, PF>{});
 }
 
 // Logical block lb of a launch over one plan; it0 is the 0-based index of
@@ -544,10 +709,13 @@ __device__ __forceinline__ void stencil3d_two_block(const Stencil2Args<T>& B, in
     // each side: wave wy's row is row wy+1
     __shared__ T ring[3][W + 2][kTileX];
     __shared__ T pring[3][W + 2][kTileX];
+    __shared__ T eg[3][2][W];  // aligned tiles: p^{t+1} just outside the tile, per row
+    __shared__ EdgeLds<T, W> EL;
     __shared__ TwoSrc<T> S;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* lds = reinterpret_cast<T*>(smem_raw);
     const StencilArgs<T>& A = B.s;
+    const int64_t nx = A.nx, ny = A.ny, nz = A.nz, halo = A.halo;
 
     // logical order: x tile fastest, then y tile, chunk, shot
     int rest = lb;
@@ -555,21 +723,35 @@ __device__ __forceinline__ void stencil3d_two_block(const Stencil2Args<T>& B, in
     const int ty = rest % A.tiles_y; rest /= A.tiles_y;
     const int chunk = rest % A.nchunks;
     const int shot = rest / A.nchunks;
+    int kbeg, kend;
+    two_chunk(B, chunk, nz, &kbeg, &kend);
+    // any z slab layer among the planes this chunk reads (block-uniform)
+    const bool zm = (A.active[2][0] && kbeg - 1 <= halo) || (A.active[2][1] && kend >= nz - halo - 1);
+    // aligned tiles where no y slab reaches the block's rows and no z slab
+    // its planes (decided per row group and chunk, so that all tiles of a
+    // row group and chunk share one tiling)
+    bool al = false;
+    if (B.align) {
+        const int64_t y0 = (int64_t)ty * (W - 2), y1 = y0 + W - 1;
+        const int64_t r0 = y0 < 1 ? 1 : y0, r1 = y1 > ny - 2 ? ny - 2 : y1;
+        const bool yslab = (A.active[1][0] && r0 <= halo) || (A.active[1][1] && r1 >= ny - halo - 1);
+        al = !yslab && !zm;
+    }
+    if (PF < 2) al = false;  // (the aligned march reads the next ring slot)
+    if (tx >= (al ? B.tiles64 : B.tiles62)) return;  // whole block: a tile of the other tiling
     const int ns = two_stage_sources(B, it0, shot, S);
     if (A.active[2][0] || A.active[2][1]) stage_march_coef(A, 2, lds);
     __syncthreads();
     const int wy = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     const int lane = (int)threadIdx.x;
-    const int64_t nx = A.nx, ny = A.ny, nz = A.nz, halo = A.halo;
-    const int64_t i = (int64_t)tx * kOwnX + lane;
+    const int64_t i = al ? (int64_t)tx * kTileX + lane : (int64_t)tx * kOwnX + lane;
     const int64_t j = (int64_t)ty * (W - 2) + wy;
     // clamped to the interior so every load stays inside the field
     const int64_t ic = i < 1 ? 1 : (i > nx - 2 ? nx - 2 : i);
     const int64_t jc = j < 1 ? 1 : (j > ny - 2 ? ny - 2 : j);
     const bool in1 = i == ic && j == jc;
-    const bool own = in1 && lane >= 1 && lane <= kOwnX && wy >= 1 && wy <= W - 2;
-    int kbeg, kend;
-    two_chunk(B, chunk, nz, &kbeg, &kend);
+    const bool orow = wy >= 1 && wy <= W - 2;
+    const bool own = in1 && orow && (al || (lane >= 1 && lane <= kOwnX));
 
     constexpr Off E = (Off)sizeof(T);
     const int64_t fo = (int64_t)shot * A.field_stride;
@@ -616,6 +798,10 @@ __device__ __forceinline__ void stencil3d_two_block(const Stencil2Args<T>& B, in
     O.xi_y = (sy == 1 ? B.xi_out[1][1] : B.xi_out[1][0]) + shot * A.xi_stride[1];
     O.xi_z[0] = B.xi_out[2][0] + shot * A.xi_stride[2];
     O.xi_z[1] = B.xi_out[2][1] + shot * A.xi_stride[2];
+    // aligned tiles store whole rows: every lane inside the row pitch
+    O.st = al && orow && j == jc && i < A.px;
+    O.in1 = in1;
+    O.scol = O.st ? ((Off)i + (Off)A.px * (Off)jc) * E : M.col;
 
     // planes holding this shot's sources
     TwoGeom G;
@@ -642,28 +828,52 @@ __device__ __forceinline__ void stencil3d_two_block(const Stencil2Args<T>& B, in
     G.iny = j == jc;
     G.dxm = lane == 0 ? (uint32_t)(0u - (uint32_t)E) : (lane == kTileX - 1 ? (uint32_t)E : 0u);
     G.dym = wy == 0 ? (uint32_t)(0u - (uint32_t)M.sy) : (wy == W - 1 ? (uint32_t)M.sy : 0u);
+    {
+        // aligned tiles: lanes 0 and 63 load for their side's node outside
+        // the tile (x = 64*tx-1, 64*tx+64; clamped into the interior, like
+        // every load); lane r < W of a halo-row wave evaluates its side's
+        // node in the block's row r (wave 0: left side, wave W-1: right)
+        G.eln = al && (lane == 0 || lane == kTileX - 1);
+        const int64_t xs = lane == 0 ? i - 1 : i + 1;
+        const int64_t xsc = xs < 1 ? 1 : (xs > nx - 2 ? nx - 2 : xs);
+        const int64_t xso = lane == 0 ? xsc - 1 : xsc + 1;
+        G.dxe = (uint32_t)((Off)(xsc - ic) * E);
+        G.dxo = (uint32_t)((Off)(xso - ic) * E);
+        const int64_t xe = wy == 0 ? (int64_t)tx * kTileX - 1 : (int64_t)tx * kTileX + kTileX;
+        const int64_t jr = (int64_t)ty * (W - 2) + (lane < W ? lane : W - 1);
+        const int64_t xec = xe < 1 ? 1 : (xe > nx - 2 ? nx - 2 : xe);
+        const int64_t jrc = jr < 1 ? 1 : (jr > ny - 2 ? ny - 2 : jr);
+        G.hcol_el = xec + A.px * jrc;
+        G.hin = xe == xec && jr == jrc;
+    }
 
     const bool xs = __any(M.sa.in);
     const bool ys = M.sb.in;  // wave-uniform
-    // any z slab layer among the planes this chunk reads (block-uniform)
-    const bool zm = (A.active[2][0] && kbeg - 1 <= halo) || (A.active[2][1] && kend >= nz - halo - 1);
-#define AWCPML_TWO(SA_, SB_, ZM_)                                                             \
-    two_march<T, Off, W, PF, SA_, SB_, ZM_>(B, M, O, ring, pring, G, S)
+#define AWCPML_TWO(SA_, SB_, ZM_, AL_)                                                        \
+    two_march<T, Off, W, PF, SA_, SB_, ZM_, AL_>(B, M, O, ring, pring, eg, &EL, G, S)
+    if constexpr (PF >= 2) {
+        if (al) {
+            // no y or z slab in the block (its rows and planes were checked)
+            if (xs) AWCPML_TWO(2, false, false, true);
+            else AWCPML_TWO(0, false, false, true);
+            return;
+        }
+    }
     if (zm) {
         if (xs) {
-            if (ys) AWCPML_TWO(2, true, true);
-            else AWCPML_TWO(2, false, true);
+            if (ys) AWCPML_TWO(2, true, true, false);
+            else AWCPML_TWO(2, false, true, false);
         } else {
-            if (ys) AWCPML_TWO(0, true, true);
-            else AWCPML_TWO(0, false, true);
+            if (ys) AWCPML_TWO(0, true, true, false);
+            else AWCPML_TWO(0, false, true, false);
         }
     } else {
         if (xs) {
-            if (ys) AWCPML_TWO(2, true, false);
-            else AWCPML_TWO(2, false, false);
+            if (ys) AWCPML_TWO(2, true, false, false);
+            else AWCPML_TWO(2, false, false, false);
         } else {
-            if (ys) AWCPML_TWO(0, true, false);
-            else AWCPML_TWO(0, false, false);
+            if (ys) AWCPML_TWO(0, true, false, false);
+            else AWCPML_TWO(0, false, false, false);
         }
     }
 #undef AWCPML_TWO

@@ -126,6 +126,13 @@ function set_time_block!(plan::HIPPlan, steps::Integer; rows::Integer=0, march::
     return plan
 end
 
+"3D two-step launches: aligned 64-node x tiles where no CPML slab reaches (bit-identical)."
+function set_two_step_align!(plan::HIPPlan, enable::Bool)
+    hipcheck(ccall((:awcpml_set_two_step_align, libawcpml), Cint, (Ptr{Cvoid}, Cint),
+                   plan.handle, enable))
+    return plan
+end
+
 "Capture the time loop in HIP graphs (replayed on later solves)."
 function set_graph!(plan::HIPPlan, enable::Bool)
     hipcheck(ccall((:awcpml_set_graph, libawcpml), Cint, (Ptr{Cvoid}, Cint),

@@ -183,6 +183,17 @@ int awcpml_set_time_block(awcpml_plan *plan, int steps, int block_rows, int marc
 int awcpml_get_time_block(const awcpml_plan *plan, int *steps, int *block_rows,
                           int *march_len, int *prefetch);
 
+/* Tiling of 3D two-step launches (enable != 0; the default for FP32 plans,
+ * off for FP64 plans): blocks whose
+ * rows and planes no CPML slab reaches own aligned 64-node x tiles, so
+ * that every row they store covers whole cache lines; the other blocks own
+ * 62-node tiles.  Applies only when no tile's outside x neighbour (x =
+ * 64*t-1 or 64*t+64) is an interior node of an active x slab.  Results
+ * are bit-identical either way.  Tuning and test support, no reference
+ * counterpart. */
+int awcpml_set_two_step_align(awcpml_plan *plan, int enable);
+int awcpml_get_two_step_align(const awcpml_plan *plan, int *enable);
+
 /* Time `nlaunches` two-step stencil launches alone (no source or receiver
  * work), in milliseconds; fails unless two-step launches apply.  The state
  * advances without sources; call awcpml_reset afterwards.  Measurement

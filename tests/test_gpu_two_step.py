@@ -5,8 +5,9 @@ A two-step launch evaluates each step with the single-step kernel's
 arithmetic, so every result must equal one step per launch bit for bit,
 in FP64 and FP32: traces, and the three field buffers afterwards.  The
 FP64 results are also compared with the CPU oracle bit for bit.  Cases
-cover several x tiles (62 owned nodes per wave) and y tiles (rows-2 owned
-rows per block), chunk lengths that leave short chunks, odd step counts
+cover several x tiles (62 owned nodes per wave, or aligned 64-node tiles
+where no CPML slab reaches a block) and y tiles (rows-2 owned rows per
+block), chunk lengths that leave short chunks, odd step counts
 (the last step runs alone), several shots, sources on different planes and
 two sources on one node, graphs, and the fallback to single steps for a
 source on a Dirichlet node.
@@ -105,6 +106,49 @@ def test_two_step_fp32_bitwise(rows):
     _run(P, o.nt, 2, rows, 0)
     two = _state(P, 1)
     _same(two, one)
+    P.close()
+
+
+# Aligned 64-node x tiles (awcpml_set_two_step_align) for the blocks no CPML
+# slab reaches: several tiles, tile counts that differ between the two
+# tilings (190: 4 tiles of 62, 3 of 64), and a grid whose tile-edge node
+# x = 191 lies in the x slab (the launch keeps 62-node tiles).  Sources and
+# receivers sit on the nodes just outside aligned tiles (0-based x = 63,
+# 64, 127, 128), which the halo-row waves evaluate for lanes 0 and 63.
+ALIGN_CASES = {
+    "multi": dict(n=(180, 61, 40), halo=6, freetop=False, nt=41),
+    "tiles_differ": dict(n=(190, 45, 30), halo=5, freetop=True, nt=37),
+    "xslab_fallback": dict(n=(197, 40, 30), halo=6, freetop=False, nt=29),
+}
+
+
+@pytest.mark.parametrize("name", list(ALIGN_CASES))
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_two_step_aligned_tiles(name, dtype):
+    print("[gpu] two-step aligned tiles %s %s" % (name, dtype), flush=True)
+    c = ALIGN_CASES[name]
+    case = small3d(n=c["n"], halo=c["halo"], freetop=c["freetop"], nt=c["nt"])
+    o = _oracle(case)
+    # 1-based positions on and next to the aligned tiles' edge nodes, in
+    # rows and planes without CPML
+    ym, zm = c["n"][1] // 2, c["n"][2] // 2
+    src = np.array([[64, ym, zm], [65, ym + 1, zm - 1], [128, ym - 2, zm + 1], [129, ym, zm]])
+    w = o.dt2srctf()
+    wav = np.asfortranarray(np.concatenate([w * (1.0 + 0.1 * e) for e in range(len(src))], axis=1))
+    P = _plan(o, dtype=dtype, srcs=[src], wav=wav)
+    rec = np.array([[64, ym, zm], [65, ym, zm], [63, ym + 3, zm], [128, ym, zm + 2], [129, ym, zm],
+                    [2, ym, zm], [c["n"][0] - 1, ym, zm]])
+    P.set_receivers(0, rec)
+    assert P.two_step_align() == (dtype == "f32")  # the default per precision
+    _run(P, o.nt, 1)
+    one = _state(P, 1)
+    for rows, march in ((16, 0), (8, 7), (4, 11)):
+        for al in (True, False):
+            P.set_two_step_align(al)
+            _run(P, o.nt, 2, rows, march)
+            assert P.time_block()[0] == 2
+            _same(_state(P, 1), one)
+    P.set_two_step_align(True)
     P.close()
 
 
