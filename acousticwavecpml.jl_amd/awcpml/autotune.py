@@ -120,9 +120,9 @@ def autotune(plan, cands=None, launches=8, cache=DEFAULT_CACHE, use_cache=True):
     # two time steps per launch, compared per time step
     from . import _lib
     two = []
-    # 3D: each shape with aligned 64-node tiles and with 62-node tiles
-    aligns = (True, False) if plan.ndim == 3 else (None,)
-    align0 = plan.two_step_align() if plan.ndim == 3 else None
+    # each shape with aligned 64-node tiles and with 62-node tiles
+    aligns = (True, False)
+    align0 = plan.two_step_align()
     for w, ml2, pf2 in (two_step_candidates(plan) if cands is None else []):
         try:
             for al in aligns:

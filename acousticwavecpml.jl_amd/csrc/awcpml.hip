@@ -578,7 +578,7 @@ awcpml::Stencil2Args<T> make_args2(const awcpml_plan* P, int64_t it0, int W, int
     B.nchunk_list = two_chunk_list(P, ml, B.chunk_start);
     B.tiles62 = A.tiles_x;
     B.tiles64 = (int)((P->n[0] + awcpml::kTileX - 1) / awcpml::kTileX);
-    B.align = P->ndim == 3 && P->tb_align && two_align_ok(P);
+    B.align = P->tb_align && two_align_ok(P);
     if (B.align) A.tiles_x = std::max(B.tiles62, B.tiles64);
     if (P->ndim == 3) {
         A.tiles_y = (int)((P->n[1] - 2 + W - 3) / (W - 2));

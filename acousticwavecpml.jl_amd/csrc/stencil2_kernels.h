@@ -386,6 +386,21 @@ __device__ __forceinline__ EdgeSlot<T> edge_slot_load(const TwoGeom& G, const Ma
     return q;
 }
 
+// 2D: the same loads for row k.
+template <typename T, typename Off, bool GEN>
+__device__ __forceinline__ EdgeSlot<T> edge_slot_load2(const TwoGeom& G, const March<T, Off, 2>& M,
+                                                       int k) {
+    EdgeSlot<T> q = {};
+    if (GEN && (k < 1 || k > G.nz - 2)) return q;
+    if (G.eln) {
+        const Off c = M.col + (Off)k * M.sm;
+        q.xo = ldg(M.pcur, c + (Off)G.dxo);
+        q.pe = ldg(M.pold, c + (Off)G.dxe);
+        q.fe = ldg(M.fact, c + (Off)G.dxe);
+    }
+    return q;
+}
+
 // LDS of the aligned tiles' outside nodes: their pcur per row over four
 // planes (slot: plane & 3), and the other step t+1 inputs of plane k
 // (slot: k & 1) -- outer x neighbour, pold, fact, inner x neighbour.
@@ -447,6 +462,9 @@ struct TwoCtx {
     int so_p, so_c, so_n;
     // aligned tiles, edge lanes: their side node's loads, PF planes in flight
     EdgeSlot<T> es[PF];
+    // 2D aligned tiles, edge lanes: their side node's pcur at row k-1 and its
+    // step t+1 result at row k-1
+    T ez0, ep1;
     // element offsets of the edge ring's slots of planes k-1, k, k+1
     int eo_p, eo_c, eo_n;
 };
@@ -924,7 +942,7 @@ __device__ __forceinline__ Mid<T> two2_first(const March<T, Off, 2>& M, int mz,
 }
 
 // Step t+2 at one node (2D, row k); stores on owned nodes only.
-template <typename T, typename Off, int SA, int ZS>
+template <typename T, typename Off, int SA, int ZS, bool AL>
 __device__ __forceinline__ void two2_second(const March<T, Off, 2>& M, const TwoOut<T, Off>& O,
                                             const Mid<T>& d, T pm, T pp, T xm, T xp, bool own,
                                             int mz, int k) {
@@ -957,8 +975,15 @@ __device__ __forceinline__ void two2_second(const March<T, Off, 2>& M, const Two
         }
     }
     const T lap = d2a + d2m;
-    if (own) stg_stream(O.p2, M.col + (Off)k * M.sm, two * pc - d.pc + f * lap + damp);
-    if (own && O.q2) stg(O.q2, M.col + (Off)k * M.sm, lap + slab);
+    if (AL) {
+        // whole rows: lanes outside the interior store the Dirichlet 0
+        const T v = two * pc - d.pc + f * lap + damp;
+        if (O.st) stg_stream(O.p2, O.scol + (Off)k * M.sm, O.in1 ? v : T(0));
+        if (O.st && O.q2) stg(O.q2, O.scol + (Off)k * M.sm, O.in1 ? lap + slab : T(0));
+    } else {
+        if (own) stg_stream(O.p2, M.col + (Off)k * M.sm, two * pc - d.pc + f * lap + damp);
+        if (own && O.q2) stg(O.q2, M.col + (Off)k * M.sm, lap + slab);
+    }
 }
 
 // Step t+1 loads of row k (2D): pp = pcur at row k+1 (every row below the
@@ -1011,29 +1036,31 @@ __device__ __forceinline__ Mid<T> two2_first_at(const TwoGeom& G, const March<T,
     return two2_first<T, Off, SA, -1>(M, mz, q, pm, pc, xm, xp);
 }
 
-template <typename T, typename Off, int SA, bool ZM>
+template <typename T, typename Off, int SA, bool ZM, bool AL>
 __device__ __forceinline__ void two2_second_at(const TwoGeom& G, const March<T, Off, 2>& M,
                                                const TwoOut<T, Off>& O, const Mid<T>& d, T pm,
                                                T pp, T xm, T xp, bool own, int k) {
     if (!ZM) {
-        two2_second<T, Off, SA, -1>(M, O, d, pm, pp, xm, xp, own, 0, k);
+        two2_second<T, Off, SA, -1, AL>(M, O, d, pm, pp, xm, xp, own, 0, k);
         return;
     }
     int mz;
     const int zs = classify(k, G.nz, G.halo, G.act, &mz);
-    if (zs == 0) two2_second<T, Off, SA, 0>(M, O, d, pm, pp, xm, xp, own, mz, k);
-    else if (zs == 1) two2_second<T, Off, SA, 1>(M, O, d, pm, pp, xm, xp, own, mz, k);
-    else two2_second<T, Off, SA, -1>(M, O, d, pm, pp, xm, xp, own, mz, k);
+    if (zs == 0) two2_second<T, Off, SA, 0, AL>(M, O, d, pm, pp, xm, xp, own, mz, k);
+    else if (zs == 1) two2_second<T, Off, SA, 1, AL>(M, O, d, pm, pp, xm, xp, own, mz, k);
+    else two2_second<T, Off, SA, -1, AL>(M, O, d, pm, pp, xm, xp, own, mz, k);
 }
 
 // One march position k (row) of a 2D wave; GEN as in two_iter.
-template <typename T, typename Off, int PF, int SA, bool ZM, bool GEN, int U>
+template <typename T, typename Off, int PF, int SA, bool ZM, bool AL, bool GEN, int U>
 __device__ __forceinline__ void two2_iter(const Stencil2Args<T>& B, const March<T, Off, 2>& M,
                                           const TwoOut<T, Off>& O, const TwoGeom& G,
                                           const TwoSrc<T>& S, TwoCtx<T, PF>& C, int k) {
     const int lane = (int)threadIdx.x;
     Mid<T> dn = {};
+    T en = T(0);  // aligned tiles, edge lanes: their side node's p^{t+1} at row k
     const T pp = own_reg(C.qr[U].pp);
+    const T epc = C.qr[U].xm;  // edge lanes: pcur at their side node, row k
     if (!GEN || (k >= 1 && k <= G.nz - 2)) {
         const T pc = C.pz1;
         const T pcx = G.inx ? pc : T(0);
@@ -1049,41 +1076,78 @@ __device__ __forceinline__ void two2_iter(const Stencil2Args<T>& B, const March<
                 if (S.lin[e] == lin) dn.p1 += S.w[e];
         }
         dn.p1 = G.in1 ? dn.p1 : T(0);
+        if (AL && G.eln) {
+            // aligned tiles: lanes 0 and 63 also evaluate step t+1 at their
+            // side's node outside the tile (x = 64*tx-1, 64*tx+64), the plain
+            // stencil in the owning wave's order of operations (no CPML slab
+            // reaches it); its pcur at row k+1 is the next ring slot's
+            // x-neighbour load, its inner x neighbour this lane's pcur
+            NodeLoads<T> q = {};
+            q.pp = C.qr[(U + 1) % PF].xm;
+            q.po = C.es[U].pe;
+            q.f = C.es[U].fe;
+            const T xo = C.es[U].xo;
+            Mid<T> e = two2_first<T, Off, 0, -1>(M, 0, q, C.ez0, epc, lane == 0 ? xo : pc,
+                                                 lane == 0 ? pc : xo);
+            if (k >= G.ksrc_lo && k <= G.ksrc_hi) {
+                const int64_t lin = G.hcol_el + (int64_t)k * G.plane;
+                for (int s = 0; s < G.ns; ++s)
+                    if (S.lin[s] == lin) e.p1 += S.w[s];
+            }
+            en = G.hin ? e.p1 : T(0);
+        }
     }
+    if (AL) C.ez0 = epc;
     C.pz0 = C.pz1;
     C.pz1 = pp;
     const int kr = k + PF;
-    if (!GEN || kr <= G.kend) C.qr[U] = two2_load<T, Off, SA, ZM, GEN>(G, M, kr);
+    if (!GEN || kr <= G.kend) {
+        C.qr[U] = two2_load<T, Off, SA, ZM, GEN>(G, M, kr);
+        if (AL) C.es[U] = edge_slot_load2<T, Off, GEN>(G, M, kr);
+    }
     __builtin_amdgcn_sched_barrier(0);
-    if (G.own && (!GEN || (k >= G.kbeg && k < G.kend))) {
+    if (AL) {
+        // whole rows (dn.p1 is 0 outside the interior)
+        if (O.st && (!GEN || (k >= G.kbeg && k < G.kend))) {
+            stg_stream(O.p1, O.scol + (Off)k * M.sm, dn.p1);
+            if (O.q1) stg(O.q1, O.scol + (Off)k * M.sm, G.in1 ? dn.q : T(0));
+        }
+    } else if (G.own && (!GEN || (k >= G.kbeg && k < G.kend))) {
         stg_stream(O.p1, M.col + (Off)k * M.sm, dn.p1);
         if (O.q1) stg(O.q1, M.col + (Off)k * M.sm, dn.q);
     }
     if (!GEN || k - 1 >= G.kbeg) {
-        const T xm = lane_shift(C.d.p1, true);
-        const T xp = lane_shift(C.d.p1, false);
-        two2_second_at<T, Off, SA, ZM>(G, M, O, C.d, C.p1m, dn.p1, xm, xp, G.own, k - 1);
+        T xm = lane_shift(C.d.p1, true);
+        T xp = lane_shift(C.d.p1, false);
+        if (AL) {
+            xm = lane == 0 ? C.ep1 : xm;
+            xp = lane == kTileX - 1 ? C.ep1 : xp;
+        }
+        two2_second_at<T, Off, SA, ZM, AL>(G, M, O, C.d, C.p1m, dn.p1, xm, xp, G.own, k - 1);
     }
     C.p1m = C.d.p1;
     C.d = dn;
+    if (AL) C.ep1 = en;
 }
 
-template <typename T, typename Off, int PF, int SA, bool ZM, bool GEN, bool CHECK, int... Us>
+template <typename T, typename Off, int PF, int SA, bool ZM, bool AL, bool GEN, bool CHECK,
+          int... Us>
 __device__ __forceinline__ void two2_group(const Stencil2Args<T>& B, const March<T, Off, 2>& M,
                                            const TwoOut<T, Off>& O, const TwoGeom& G,
                                            const TwoSrc<T>& S, TwoCtx<T, PF>& C, int k,
                                            std::integer_sequence<int, Us...>) {
     ((!CHECK || k + Us <= G.kend
-          ? (two2_iter<T, Off, PF, SA, ZM, GEN, Us % PF>(B, M, O, G, S, C, k + Us), 0)
+          ? (two2_iter<T, Off, PF, SA, ZM, AL, GEN, Us % PF>(B, M, O, G, S, C, k + Us), 0)
           : 0),
      ...);
 }
 
 // The march of one 2D wave: head, steady state, tail as in two_march.
-template <typename T, typename Off, int PF, int SA, bool ZM>
+template <typename T, typename Off, int PF, int SA, bool ZM, bool AL>
 __device__ __forceinline__ void two2_march(const Stencil2Args<T>& B, const March<T, Off, 2>& M,
                                            const TwoOut<T, Off>& O, const TwoGeom& G,
                                            const TwoSrc<T>& S) {
+    static_assert(!AL || PF >= 2, "aligned tiles read the next ring slot");
     TwoCtx<T, PF> C;
     C.pz0 = G.kbeg >= 2 ? ldg(M.pcur, M.col + (Off)(G.kbeg - 2) * M.sm) : T(0);
     C.pz1 = ldg(M.pcur, M.col + (Off)(G.kbeg - 1) * M.sm);
@@ -1091,25 +1155,31 @@ __device__ __forceinline__ void two2_march(const Stencil2Args<T>& B, const March
     for (int u = 0; u < PF; ++u) {
         const int kk = G.kbeg - 1 + u;
         C.qr[u] = kk <= G.kend ? two2_load<T, Off, SA, ZM, true>(G, M, kk) : NodeLoads<T>{};
+        if (AL) C.es[u] = kk <= G.kend ? edge_slot_load2<T, Off, true>(G, M, kk) : EdgeSlot<T>{};
         __builtin_amdgcn_sched_barrier(0);
     }
+    C.ez0 = T(0);
+    if (AL && G.eln && G.kbeg >= 2)
+        C.ez0 = ldg(M.pcur, M.col + (Off)(G.kbeg - 2) * M.sm + (Off)G.dxe);
+    C.ep1 = T(0);
     C.d = Mid<T>{};
     C.p1m = T(0);
     C.slot = 0;
     constexpr int H = (2 + PF - 1) / PF * PF;
     int k = G.kbeg - 1;
-    two2_group<T, Off, PF, SA, ZM, true, true>(B, M, O, G, S, C, k, std::make_integer_sequence<int, H>{});
+    two2_group<T, Off, PF, SA, ZM, AL, true, true>(B, M, O, G, S, C, k,
+                                                   std::make_integer_sequence<int, H>{});
     k += H;
     const int last = (G.kend < G.nz - 2 ? G.kend : G.nz - 2) - PF;
     // drain the head's memory operations once, as in two_march, so that the
     // steady loop's waits follow its own issue order
     __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
     for (; k + PF - 1 <= last; k += PF)
-        two2_group<T, Off, PF, SA, ZM, false, false>(B, M, O, G, S, C, k,
-                                                     std::make_integer_sequence<int, PF>{});
+        two2_group<T, Off, PF, SA, ZM, AL, false, false>(B, M, O, G, S, C, k,
+                                                         std::make_integer_sequence<int, PF>{});
     for (; k <= G.kend; k += PF)
-        two2_group<T, Off, PF, SA, ZM, true, true>(B, M, O, G, S, C, k,
-                                                   std::make_integer_sequence<int, PF>{});
+        two2_group<T, Off, PF, SA, ZM, AL, true, true>(B, M, O, G, S, C, k,
+                                                       std::make_integer_sequence<int, PF>{});
 }
 
 template <typename T, typename Off, int BY, int PF>
@@ -1132,12 +1202,18 @@ __device__ __forceinline__ void stencil2d_two_block(const Stencil2Args<T>& B, in
     if (chunk >= A.nchunks) return;  // no barrier follows
     const int lane = (int)threadIdx.x;
     const int64_t nx = A.nx, ny = A.ny, halo = A.halo;
-    const int64_t i = (int64_t)tx * kOwnX + lane;
-    const int64_t ic = i < 1 ? 1 : (i > nx - 2 ? nx - 2 : i);
-    const bool in1 = i == ic;
-    const bool own = in1 && lane >= 1 && lane <= kOwnX;
     int kbeg, kend;
     two_chunk(B, chunk, ny, &kbeg, &kend);
+    // any y slab row among the rows this chunk reads (wave-uniform)
+    const bool zm = (A.active[1][0] && kbeg - 1 <= halo) || (A.active[1][1] && kend >= ny - halo - 1);
+    // aligned 64-node tiles for chunks no y slab reaches (decided per chunk,
+    // so that all tiles of a chunk share one tiling)
+    const bool al = PF >= 2 && B.align && !zm;
+    if (tx >= (al ? B.tiles64 : B.tiles62)) return;  // a tile of the other tiling
+    const int64_t i = al ? (int64_t)tx * kTileX + lane : (int64_t)tx * kOwnX + lane;
+    const int64_t ic = i < 1 ? 1 : (i > nx - 2 ? nx - 2 : i);
+    const bool in1 = i == ic;
+    const bool own = in1 && (al || (lane >= 1 && lane <= kOwnX));
 
     constexpr Off E = (Off)sizeof(T);
     const int64_t fo = (int64_t)shot * A.field_stride;
@@ -1182,6 +1258,10 @@ __device__ __forceinline__ void stencil2d_two_block(const Stencil2Args<T>& B, in
     O.xi_y = nullptr;
     O.xi_z[0] = B.xi_out[1][0] + shot * A.xi_stride[1];
     O.xi_z[1] = B.xi_out[1][1] + shot * A.xi_stride[1];
+    // aligned tiles store whole rows: every lane inside the row pitch
+    O.st = al && i < A.px;
+    O.in1 = in1;
+    O.scol = O.st ? (Off)i * E : M.col;
 
     // rows holding this shot's sources
     TwoGeom G;
@@ -1206,15 +1286,35 @@ __device__ __forceinline__ void stencil2d_two_block(const Stencil2Args<T>& B, in
     G.own = own;
     G.inx = in1;
     G.iny = true;
+    {
+        // aligned tiles: lanes 0 and 63 also evaluate their side's node just
+        // outside the tile (x = 64*tx-1, 64*tx+64; loads clamped into the
+        // interior like every load)
+        G.eln = al && (lane == 0 || lane == kTileX - 1);
+        const int64_t xe = lane == 0 ? i - 1 : i + 1;
+        const int64_t xec = xe < 1 ? 1 : (xe > nx - 2 ? nx - 2 : xe);
+        const int64_t xeo = lane == 0 ? xec - 1 : xec + 1;
+        G.dxe = (uint32_t)((Off)(xec - ic) * E);
+        G.dxo = (uint32_t)((Off)(xeo - ic) * E);
+        G.hcol_el = xec;
+        G.hin = xe == xec;
+    }
 
     const bool xs = __any(M.sa.in);
-    const bool zm = (A.active[1][0] && kbeg - 1 <= halo) || (A.active[1][1] && kend >= ny - halo - 1);
+    if constexpr (PF >= 2) {
+        if (al) {
+            // no y slab in the chunk
+            if (xs) two2_march<T, Off, PF, 2, false, true>(B, M, O, G, S);
+            else two2_march<T, Off, PF, 0, false, true>(B, M, O, G, S);
+            return;
+        }
+    }
     if (xs) {
-        if (zm) two2_march<T, Off, PF, 2, true>(B, M, O, G, S);
-        else two2_march<T, Off, PF, 2, false>(B, M, O, G, S);
+        if (zm) two2_march<T, Off, PF, 2, true, false>(B, M, O, G, S);
+        else two2_march<T, Off, PF, 2, false, false>(B, M, O, G, S);
     } else {
-        if (zm) two2_march<T, Off, PF, 0, true>(B, M, O, G, S);
-        else two2_march<T, Off, PF, 0, false>(B, M, O, G, S);
+        if (zm) two2_march<T, Off, PF, 0, true, false>(B, M, O, G, S);
+        else two2_march<T, Off, PF, 0, false, false>(B, M, O, G, S);
     }
 }
 

@@ -255,6 +255,42 @@ def test_two_step_2d_fp64_bitwise(name, rows, march):
     P.close()
 
 
+# 2D aligned tiles: lanes 0 and 63 of every wave evaluate their row's
+# tile-edge nodes; sources and receivers on those nodes (0-based x = 63,
+# 64, 127, 128), a grid with an x slab over a tile edge (x = 191: the launch
+# keeps 62-node tiles) and tile counts that differ between the tilings.
+ALIGN_CASES_2D = {
+    "multi": dict(n=(180, 71), halo=6, freetop=False, nt=45),
+    "tiles_differ": dict(n=(190, 57), halo=5, freetop=True, nt=41),
+    "xslab_fallback": dict(n=(197, 50), halo=6, freetop=False, nt=33),
+}
+
+
+@pytest.mark.parametrize("name", list(ALIGN_CASES_2D))
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_two_step_2d_aligned_tiles(name, dtype):
+    print("[gpu] two-step 2D aligned tiles %s %s" % (name, dtype), flush=True)
+    o = _oracle(_interior_2d(**ALIGN_CASES_2D[name]))
+    n = ALIGN_CASES_2D[name]["n"]
+    ym = n[1] // 2
+    src = np.array([[64, ym], [65, ym + 1], [128, ym - 2], [129, ym]])
+    w = o.dt2srctf()
+    wav = np.asfortranarray(np.concatenate([w * (1.0 + 0.1 * e) for e in range(len(src))], axis=1))
+    P = _plan(o, dtype=dtype, srcs=[src], wav=wav)
+    P.set_receivers(0, np.array([[64, ym], [65, ym + 3], [128, ym], [129, ym - 1], [2, ym],
+                                 [n[0] - 1, ym]]))
+    assert P.two_step_align() == (dtype == "f32")
+    _run(P, o.nt, 1)
+    one = _state(P, 1)
+    for rows, march in ((4, 0), (8, 5), (4, 13)):
+        for al in (True, False):
+            P.set_two_step_align(al)
+            _run(P, o.nt, 2, rows, march)
+            assert P.time_block()[0] == 2
+            _same(_state(P, 1), one)
+    P.close()
+
+
 def test_two_step_2d_fp32_shots_graphs():
     print("[gpu] two-step 2D fp32, 3 shots, graphs", flush=True)
     o = _oracle(_interior_2d(n=(150, 61), halo=6, freetop=False, nt=73))
