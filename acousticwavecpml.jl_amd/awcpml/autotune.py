@@ -12,7 +12,9 @@ reset afterwards.
 Multi-shot plans also try shot groups (the waves of a block march the same
 rows of several shots, sharing the model rows they read).  Two-step
 launches (``Plan.set_time_block``: two time steps per launch, bit-identical
-results) are timed per time step too, and kept when they are faster.
+results) are timed per time step too, each shape with aligned 64-node x
+tiles and with 62-node tiles (``Plan.set_two_step_align``), and kept when
+they are faster; the choice is recorded as ``time_block["align"]``.
 """
 import json
 import os
