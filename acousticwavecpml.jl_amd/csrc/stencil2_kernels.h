@@ -287,7 +287,8 @@ __device__ __forceinline__ void two_second(const March<T, Off, 3>& M, const TwoO
         if (O.st) stg_stream(O.p2, O.scol + (Off)k * M.sm, O.in1 ? v : T(0));
         if (O.st && O.q2) stg(O.q2, O.scol + (Off)k * M.sm, O.in1 ? lap + slab : T(0));
     } else {
-        if (own) stg_stream(O.p2, M.col + (Off)k * M.sm, two * pc - d.pc + f * lap + damp);
+        // 62-node tiles: plain stores (see the p^{t+1} store)
+        if (own) stg(O.p2, M.col + (Off)k * M.sm, two * pc - d.pc + f * lap + damp);
         if (own && O.q2) stg(O.q2, M.col + (Off)k * M.sm, lap + slab);
     }
 }
@@ -571,7 +572,9 @@ __device__ __forceinline__ void two_iter(const Stencil2Args<T>& B, const March<T
             if (O.q1) stg(O.q1, O.scol + (Off)k * M.sm, G.in1 ? dn.q : T(0));
         }
     } else if (G.own && (!GEN || (k >= G.kbeg && k < G.kend))) {
-        stg_stream(O.p1, M.col + (Off)k * M.sm, dn.p1);
+        // 62-node tiles: rows end in cache lines the neighbouring tile's
+        // block completes; plain stores let L2 merge the two halves
+        stg(O.p1, M.col + (Off)k * M.sm, dn.p1);
         if (O.q1) stg(O.q1, M.col + (Off)k * M.sm, dn.q);
     }
     __syncthreads();
