@@ -103,6 +103,11 @@ struct awcpml_plan {
     int64_t field_elems = 0;
     void* p[3] = {nullptr, nullptr, nullptr};
     void* fact = nullptr;
+    // fact at the aligned two-step tiles' edge nodes (x = 64*t-1, 64*t+64,
+    // clamped into the interior), rows of a tile edge contiguous:
+    // 3D [z][t][side][y], 2D [y][t][side]
+    void* fact_edge = nullptr;
+    int64_t fe_elems = 0;
     void* psi[3][2][2] = {};
     void* xi[3][2] = {};
     int64_t psi_elems[3] = {0, 0, 0};
@@ -578,6 +583,7 @@ awcpml::Stencil2Args<T> make_args2(const awcpml_plan* P, int64_t it0, int W, int
     B.nchunk_list = two_chunk_list(P, ml, B.chunk_start);
     B.tiles62 = A.tiles_x;
     B.tiles64 = (int)((P->n[0] + awcpml::kTileX - 1) / awcpml::kTileX);
+    B.fact_edge = (const T*)P->fact_edge;
     B.align = P->tb_align && two_align_ok(P);
     if (B.align) A.tiles_x = std::max(B.tiles62, B.tiles64);
     if (P->ndim == 3) {
@@ -1272,6 +1278,11 @@ int awcpml_plan_create(int ndim, int64_t nx, int64_t ny, int64_t nz, int halo, i
     const size_t fbytes = (size_t)P->field_elems * nshots * P->esize;
     for (int b = 0; b < 3 && !st; ++b) st = dalloc(&P->p[b], fbytes);
     if (!st) st = dalloc(&P->fact, (size_t)P->field_elems * P->esize);
+    {
+        const int64_t tiles = (nx + awcpml::kTileX - 1) / awcpml::kTileX;
+        P->fe_elems = ndim == 3 ? nz * tiles * 2 * ny : ny * tiles * 2;
+        if (!st) st = dalloc(&P->fact_edge, (size_t)P->fe_elems * P->esize);
+    }
     for (int a = 0; a < ndim && !st; ++a) {
         P->psi_elems[a] = slab_elems(P, a, halo + 1);
         P->xi_elems[a] = slab_elems(P, a, halo);
@@ -1284,6 +1295,8 @@ int awcpml_plan_create(int ndim, int64_t nx, int64_t ny, int64_t nz, int halo, i
     }
     if (!st) {
         hipError_t e = hipMemsetAsync(P->fact, 0, (size_t)P->field_elems * P->esize, P->stream);
+        if (e == hipSuccess)
+            e = hipMemsetAsync(P->fact_edge, 0, (size_t)P->fe_elems * P->esize, P->stream);
         if (e != hipSuccess) st = fail(AWCPML_ERR_HIP, "hipMemset failed: %s", hipGetErrorString(e));
     }
     P->home = get_layout(P);
@@ -1306,6 +1319,7 @@ int awcpml_plan_destroy(awcpml_plan* P) {
     for (int b = 0; b < 3; ++b) dfree(P->p[b]);
     dfree(P->p_spare);
     dfree(P->fact);
+    dfree(P->fact_edge);
     for (int a = 0; a < 3; ++a) {
         for (int s = 0; s < 2; ++s) {
             dfree(P->psi[a][s][0]);
@@ -1340,14 +1354,34 @@ int awcpml_set_model(awcpml_plan* P, const double* fact, const double* inv_d, co
         P->inv_d2[a] = inv_d2[a];
     }
     const int64_t nx = P->n[0], rows = P->n[1] * P->n[2];
+    // the aligned tiles' edge-node copy: same values, rows of an edge adjacent
+    const int64_t ny = P->n[1], nz = P->ndim == 3 ? P->n[2] : 1;
+    const int64_t tiles = (nx + awcpml::kTileX - 1) / awcpml::kTileX;
+    std::vector<double> fe((size_t)P->fe_elems);
+    for (int64_t k = 0; k < (P->ndim == 3 ? nz : ny); ++k)
+        for (int64_t t = 0; t < tiles; ++t)
+            for (int s = 0; s < 2; ++s) {
+                int64_t x = s == 0 ? t * awcpml::kTileX - 1 : t * awcpml::kTileX + awcpml::kTileX;
+                x = x < 1 ? 1 : (x > nx - 2 ? nx - 2 : x);
+                if (P->ndim == 3) {
+                    for (int64_t j = 0; j < ny; ++j)
+                        fe[(size_t)(((k * tiles + t) * 2 + s) * ny + j)] = fact[x + nx * (j + ny * k)];
+                } else {
+                    fe[(size_t)((k * tiles + t) * 2 + s)] = fact[x + nx * k];
+                }
+            }
     if (P->dtype == AWCPML_F64) {
         HIP_TRY(hipMemcpy2DAsync(P->fact, P->px * 8, fact, nx * 8, nx * 8, rows, hipMemcpyHostToDevice,
                                  P->stream));
+        HIP_TRY(hipMemcpyAsync(P->fact_edge, fe.data(), fe.size() * 8, hipMemcpyHostToDevice, P->stream));
     } else {
         std::vector<float> tmp((size_t)nx * rows);
         for (size_t q = 0; q < tmp.size(); ++q) tmp[q] = (float)fact[q];
         HIP_TRY(hipMemcpy2DAsync(P->fact, P->px * 4, tmp.data(), nx * 4, nx * 4, rows, hipMemcpyHostToDevice,
                                  P->stream));
+        std::vector<float> fe32(fe.size());
+        for (size_t q = 0; q < fe.size(); ++q) fe32[q] = (float)fe[q];
+        HIP_TRY(hipMemcpyAsync(P->fact_edge, fe32.data(), fe32.size() * 4, hipMemcpyHostToDevice, P->stream));
     }
     HIP_TRY(hipStreamSynchronize(P->stream));
     P->model_set = true;

@@ -79,6 +79,10 @@ struct Stencil2Args {
     // two tilings' tiles
     int align;
     int tiles62, tiles64;    // x tiles of the 62-lane and the aligned tiling
+    // fact at the aligned tiles' edge nodes, rows of an edge adjacent: 3D
+    // [z][tile][side][y], 2D [y][tile][side] (the edge lanes' fact loads then
+    // share cache lines across the rows of a block)
+    const T* fact_edge;
     // march chunks: chunk c owns planes (2D: rows) [chunk_start[c],
     // chunk_start[c+1]); 0 chunks listed: chunk c starts at 1 + c*march_len.
     // The host cuts the CPML layers of the march axis into chunks of their
@@ -363,6 +367,9 @@ struct TwoGeom {
     uint32_t dxo;     // ... and to that node's outer x neighbour
     int64_t hcol_el;  // halo-row waves: element offset of lane r's node column (sources)
     bool hin;         // ... that node is interior
+    const void* fedge;  // the plan's edge copy of fact
+    uint32_t fe_off;    // byte offset of this edge lane's entry at march position 0
+    uint32_t fe_step;   // bytes per march position
 };
 
 // Step t+1 inputs of the node just outside an aligned tile that its row's
@@ -382,7 +389,7 @@ __device__ __forceinline__ EdgeSlot<T> edge_slot_load(const TwoGeom& G, const Ma
         const Off c = M.col + (Off)k * M.sm;
         q.xo = ldg(M.pcur, c + (Off)G.dxo);
         q.pe = ldg(M.pold, c + (Off)G.dxe);
-        q.fe = ldg(M.fact, c + (Off)G.dxe);
+        q.fe = ldg((const T*)G.fedge, (Off)G.fe_off + (Off)k * (Off)G.fe_step);
     }
     return q;
 }
@@ -397,7 +404,7 @@ __device__ __forceinline__ EdgeSlot<T> edge_slot_load2(const TwoGeom& G, const M
         const Off c = M.col + (Off)k * M.sm;
         q.xo = ldg(M.pcur, c + (Off)G.dxo);
         q.pe = ldg(M.pold, c + (Off)G.dxe);
-        q.fe = ldg(M.fact, c + (Off)G.dxe);
+        q.fe = ldg((const T*)G.fedge, (Off)G.fe_off + (Off)k * (Off)G.fe_step);
     }
     return q;
 }
@@ -866,6 +873,10 @@ __device__ __forceinline__ void stencil3d_two_block(const Stencil2Args<T>& B, in
         const int64_t jrc = jr < 1 ? 1 : (jr > ny - 2 ? ny - 2 : jr);
         G.hcol_el = xec + A.px * jrc;
         G.hin = xe == xec && jr == jrc;
+        // this edge lane's entry of the edge copy of fact: [z][tile][side][y]
+        G.fedge = B.fact_edge;
+        G.fe_off = (uint32_t)((((Off)tx * 2 + (lane == 0 ? 0 : 1)) * (Off)ny + (Off)jc) * E);
+        G.fe_step = (uint32_t)((Off)B.tiles64 * 2 * (Off)ny * E);
     }
 
     const bool xs = __any(M.sa.in);
@@ -1301,6 +1312,10 @@ __device__ __forceinline__ void stencil2d_two_block(const Stencil2Args<T>& B, in
         G.dxo = (uint32_t)((Off)(xeo - ic) * E);
         G.hcol_el = xec;
         G.hin = xe == xec;
+        // this edge lane's entry of the edge copy of fact: [y][tile][side]
+        G.fedge = B.fact_edge;
+        G.fe_off = (uint32_t)(((Off)tx * 2 + (lane == 0 ? 0 : 1)) * E);
+        G.fe_step = (uint32_t)((Off)B.tiles64 * 2 * E);
     }
 
     const bool xs = __any(M.sa.in);
